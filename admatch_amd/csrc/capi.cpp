@@ -1,0 +1,269 @@
+// C ABI (include/admatch.h) over the Engine.
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/admatch.h"
+#include "engine.hpp"
+
+using admatch::Engine;
+
+struct ita_handle {
+    uint16_t flags = 0;
+    int32_t kind = admatch::KIND_ENGINE;
+    int32_t mode = admatch::MODE_ADMATCH;
+    Engine eng;
+    std::vector<int32_t> off, ids;  // reused result buffers
+};
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int rc, const std::string& msg) {
+    g_err = msg;
+    return rc;
+}
+
+int check(ita_handle* h) {
+    if (!h) return fail(ITA_ERR_ARG, "null handle");
+    return ITA_OK;
+}
+
+// kernel / selector from the reference's flag bits (Interface.cu:12-40)
+int resolve(uint16_t flags, int32_t* kind, int32_t* mode) {
+    const int k = ((flags & ITA_INFA_KERNEL) != 0) + ((flags & ITA_AS_KERNEL) != 0) + ((flags & ITA_TKO_KERNEL) != 0);
+    if (flags & ITA_AD_MATCHER) {
+        if (k != 0) return fail(ITA_ERR_ARG, "AD_MATCHER takes no kernel bit");
+        *kind = admatch::KIND_ENGINE;
+        if ((flags & ITA_LEARNED_SELECTOR) && (flags & ITA_HEURISTIC_SELECTOR))
+            return fail(ITA_ERR_ARG, "choose one selector");
+        *mode = (flags & ITA_LEARNED_SELECTOR)     ? admatch::MODE_LEARNED
+                : (flags & ITA_HEURISTIC_SELECTOR) ? admatch::MODE_HEUR
+                                                   : admatch::MODE_ADMATCH;
+        return ITA_OK;
+    }
+    if (flags & (ITA_LEARNED_SELECTOR | ITA_HEURISTIC_SELECTOR)) return fail(ITA_ERR_ARG, "selector bits need AD_MATCHER");
+    if (k != 1) return fail(ITA_ERR_ARG, "exactly one kernel bit required");
+    if (flags & ITA_INFA_KERNEL) {
+        *kind = admatch::KIND_PLAIN;
+        *mode = admatch::MODE_INFA;
+    } else {
+        *kind = admatch::KIND_ENGINE;
+        *mode = (flags & ITA_TKO_KERNEL) ? admatch::MODE_TKO : admatch::MODE_AS;
+    }
+    return ITA_OK;
+}
+
+int status(int rc, const std::string& err) {
+    if (rc != 0) g_err = err;
+    return rc;
+}
+
+void maybe_report(ita_handle* h, int32_t n, double wall_ms) {
+    if (h->flags & ITA_PROFILER_MODE) {
+        const auto& st = h->eng.stats();
+        std::printf("%20s%10.4f ms\n%20s%10.6f ms\n%20s%10.4f ms\n%20s%10.6f ms\n", "Total: ", wall_ms,
+                    "Per-string: ", n ? wall_ms / n : 0.0, "Kernel Exec: ", double(st.kernel_ms),
+                    "Per-string Kernel: ", n ? double(st.kernel_ms) / n : 0.0);
+    }
+    if (h->flags & ITA_SHOW_RESULTS) {
+        for (int32_t i = 0; i < n; ++i) {
+            std::printf("==================\nString [%d] :\nAccept Rules : ", i + 1);
+            for (int32_t k = h->off[i]; k < h->off[i + 1]; ++k) std::printf("%d ", h->ids[k]);
+            std::printf("\n==================\n\n");
+        }
+    }
+    std::fflush(stdout);
+}
+}  // namespace
+
+extern "C" {
+
+const char* ita_last_error(void) { return g_err.c_str(); }
+
+int ita_create(uint16_t flags, const char* nfa_path, ita_handle** out) {
+    if (!out || !nfa_path) return fail(ITA_ERR_ARG, "null argument");
+    *out = nullptr;
+    int32_t kind, mode;
+    if (int rc = resolve(flags, &kind, &mode)) return rc;
+    auto* h = new ita_handle();
+    h->flags = flags;
+    h->kind = kind;
+    h->mode = mode;
+    std::string err;
+    if (int rc = h->eng.load(nfa_path, err)) {
+        delete h;
+        return fail(rc, err);
+    }
+    *out = h;
+    return ITA_OK;
+}
+
+int ita_alloc(ita_handle* h) {
+    if (int rc = check(h)) return rc;
+    std::string err;
+    return status(h->eng.upload(err), err);
+}
+
+int ita_free(ita_handle* h) {
+    if (int rc = check(h)) return rc;
+    h->eng.release();
+    return ITA_OK;
+}
+
+int ita_destroy(ita_handle* h) {
+    delete h;
+    return ITA_OK;
+}
+
+int ita_batched_scan(ita_handle* h, const uint8_t* const* text, const int32_t* len, int32_t n,
+                     int32_t* out_offsets, int32_t* out_ids, int64_t cap) {
+    if (int rc = check(h)) return rc;
+    if (n < 0 || (n > 0 && (!out_offsets || (cap > 0 && !out_ids)))) return fail(ITA_ERR_ARG, "bad output buffers");
+    std::string err;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc = h->eng.scan_host(text, len, n, admatch::KernelKind(h->kind), h->mode, h->off, h->ids, err);
+    if (rc) return fail(rc, err);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (out_offsets) std::memcpy(out_offsets, h->off.data(), sizeof(int32_t) * (size_t(n) + 1));
+    const int64_t total = h->off.empty() ? 0 : h->off.back();
+    if (total > cap) return fail(ITA_ERR_CAPACITY, "out_ids too small");
+    if (total) std::memcpy(out_ids, h->ids.data(), sizeof(int32_t) * size_t(total));
+    maybe_report(h, n, ms);
+    return ITA_OK;
+}
+
+int ita_scan(ita_handle* h, const char* text, int32_t* out_ids, int32_t cap, int32_t* n_out) {
+    if (int rc = check(h)) return rc;
+    if (!text) return fail(ITA_ERR_ARG, "null text");
+    const uint8_t* p = reinterpret_cast<const uint8_t*>(text);
+    const int32_t len = int32_t(std::strlen(text));  // Interface.cu:155
+    int32_t off[2];
+    int rc = ita_batched_scan(h, &p, &len, 1, off, cap > 0 ? out_ids : nullptr, cap);
+    if (n_out) *n_out = off[1];
+    return rc;
+}
+
+int ita_scan_device(ita_handle* h, const uint8_t* d_text, const int64_t* d_offsets, int32_t n, int32_t kind,
+                    int32_t mode, void* stream, int32_t use_graph) {
+    if (int rc = check(h)) return rc;
+    if (kind != ITA_KIND_PLAIN && kind != ITA_KIND_ENGINE) return fail(ITA_ERR_ARG, "bad kind");
+    if (mode < ITA_MODE_INFA || mode > ITA_MODE_LOG) return fail(ITA_ERR_ARG, "bad mode");
+    std::string err;
+    return status(h->eng.scan_device(d_text, d_offsets, n, admatch::KernelKind(kind), mode,
+                                     static_cast<hipStream_t>(stream), use_graph != 0, err),
+                  err);
+}
+
+int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids, int64_t cap) {
+    if (int rc = check(h)) return rc;
+    if (n < 0 || !out_offsets) return fail(ITA_ERR_ARG, "bad output buffers");
+    std::string err;
+    if (int rc = h->eng.collect(n, h->off, h->ids, nullptr, err)) return fail(rc, err);
+    std::memcpy(out_offsets, h->off.data(), sizeof(int32_t) * (size_t(n) + 1));
+    const int64_t total = h->off.back();
+    if (total > cap) return fail(ITA_ERR_CAPACITY, "out_ids too small");
+    if (total) std::memcpy(out_ids, h->ids.data(), sizeof(int32_t) * size_t(total));
+    return ITA_OK;
+}
+
+int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3) {
+    if (int rc = check(h)) return rc;
+    const auto& st = h->eng.stats();
+    if (kernel_ms) *kernel_ms = st.kernel_ms;
+    if (method_steps3)
+        for (int i = 0; i < 3; ++i) method_steps3[i] = st.method_steps[i];
+    return ITA_OK;
+}
+
+int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs) {
+    if (int rc = check(h)) return rc;
+    if ((wpb != 1 && wpb != 2 && wpb != 4) || (cap != 256 && cap != 512 && cap != 1024) ||
+        (plain_bs != 64 && plain_bs != 128 && plain_bs != 256))
+        return fail(ITA_ERR_ARG, "unsupported variant");
+    admatch::Variant v;
+    v.wpb = wpb;
+    v.cap = cap;
+    v.plain_bs = plain_bs;
+    h->eng.set_variant(v);
+    return ITA_OK;
+}
+
+int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs) {
+    if (int rc = check(h)) return rc;
+    const auto& v = h->eng.variant();
+    if (wpb) *wpb = v.wpb;
+    if (cap) *cap = v.cap;
+    if (plain_bs) *plain_bs = v.plain_bs;
+    return ITA_OK;
+}
+
+int ita_set_selector(ita_handle* h, const float* weights, int32_t n, int32_t every) {
+    if (int rc = check(h)) return rc;
+    std::string err;
+    return status(h->eng.set_mlp(weights, n, every, err), err);
+}
+
+int ita_set_log(ita_handle* h, int64_t cap_records, int32_t stride) {
+    if (int rc = check(h)) return rc;
+    std::string err;
+    return status(h->eng.set_log(cap_records, stride, err), err);
+}
+
+int64_t ita_fetch_log(ita_handle* h, float* out, int64_t cap_records) {
+    if (int rc = check(h)) return rc;
+    std::string err;
+    const int64_t n = h->eng.fetch_log(out, cap_records, err);
+    if (n < 0) g_err = err;
+    return n;
+}
+
+int ita_info(ita_handle* h, int64_t* out) {
+    if (int rc = check(h)) return rc;
+    if (!out) return fail(ITA_ERR_ARG, "null output");
+    const auto& g = h->eng.graph();
+    out[0] = g.S;
+    out[1] = g.W;
+    out[2] = g.sym_off[admatch::kSymbols];
+    out[3] = g.wb_count();
+    out[4] = int64_t(g.tko_shift.size());
+    out[5] = g.tko_threshold;
+    out[6] = int64_t(g.as_dst.size());
+    out[7] = int64_t(h->eng.engine_lds_bytes(h->eng.variant()));
+    out[8] = int64_t(h->eng.plain_lds_bytes());
+    out[9] = h->eng.device_cu_count();
+    out[10] = int64_t(g.content_hash());
+    return ITA_OK;
+}
+
+int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap) {
+    if (int rc = check(h)) return rc;
+    const auto& g = h->eng.graph();
+    auto give = [&](const void* src, int64_t count, size_t elem) -> int64_t {
+        if (out && cap > 0) std::memcpy(out, src, size_t(count < cap ? count : cap) * elem);
+        return count;
+    };
+    switch (which) {
+        case 0: return give(g.sym_off.data(), int64_t(g.sym_off.size()), 4);
+        case 1: return give(g.list.data(), int64_t(g.list.size()) * 2, 4);
+        case 2: return give(g.tko_koff.data(), int64_t(g.tko_koff.size()), 4);
+        case 3: return give(g.tko_shift.data(), int64_t(g.tko_shift.size()), 4);
+        case 4: return give(g.tko_lim.data(), int64_t(g.tko_lim.size()), 8);
+        case 5: return give(g.tko_loff.data(), int64_t(g.tko_loff.size()), 4);
+        case 6: return give(g.tko_left.data(), int64_t(g.tko_left.size()) * 2, 4);
+        case 7: return give(g.as_off.data(), int64_t(g.as_off.size()), 4);
+        case 8: return give(g.as_dst.data(), int64_t(g.as_dst.size()), 4);
+        case 9: {
+            std::vector<uint64_t> v;
+            v.insert(v.end(), g.init.begin(), g.init.end());
+            v.insert(v.end(), g.persis.begin(), g.persis.end());
+            v.insert(v.end(), g.accept.begin(), g.accept.end());
+            return give(v.data(), int64_t(v.size()), 8);
+        }
+        default: return fail(ITA_ERR_ARG, "bad table id");
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// Device-side types and wave64 helpers shared by the scan kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace admatch {
+
+// Read-only compiled rule set as the kernels see it (all pointers device).
+struct DevGraph {
+    int32_t S, W;           // states, u64 words per state vector
+    const uint64_t* init;   // [W]
+    const uint64_t* persis; // [W]
+    const uint64_t* accept; // [W]
+    const int32_t* sym_off; // [258] iNFA CSR offsets
+    const uint2* list;      // (src,dst) per symbol, sorted
+    const int32_t* tko_koff;  // [258]
+    const int32_t* tko_shift; // [K]
+    const uint64_t* tko_lim;  // [K*W]
+    const int32_t* tko_loff;  // [258]
+    const uint2* tko_left;    // leftover (src,dst)
+    const uint32_t* as_off;   // [S*256+1]
+    const uint32_t* as_dst;
+    const int4* sym_stats;    // [256]: |T_c|, k_c, |leftover_c|, 0
+    int32_t wb_begin, wb_count;  // word-boundary slice of list
+};
+
+// Per-launch arguments of the batched engine.
+struct ScanArgs {
+    const uint8_t* text;      // concatenated strings
+    const int64_t* offsets;   // [n+1]
+    const int32_t* order;     // [n] processing order (longest first) or null
+    int32_t n;
+    int32_t mode;             // step selector, see SelectMode
+    int32_t* queue;           // [1] work counter (zeroed before launch)
+    int32_t* res_count;       // [n] accepted-state count per string
+    int64_t* res_start;       // [n] start in res_pool
+    uint32_t* res_pool;       // accepted state ids, ascending per string
+    int64_t pool_cap;
+    unsigned long long* pool_top;  // [1] (zeroed before launch)
+    int32_t* overflow;        // [1] set when the pool is exhausted
+    const float* mlp;         // learned selector weights or null
+    int32_t mlp_every;        // evaluate the selector every k steps
+    // step logging (MODE_LOG): per sampled step 16 features + 3 times
+    float* log_buf;
+    int64_t log_cap;
+    unsigned long long* log_top;
+    int32_t log_stride;       // sample one step in log_stride
+    unsigned long long* method_steps;  // [4] steps per method, or null
+};
+
+enum SelectMode : int32_t {
+    MODE_INFA = 0,     // engine, iNFA scatter every step
+    MODE_TKO = 1,      // engine, TKO every step
+    MODE_AS = 2,       // engine, active-state rows every step (iNFA when dense)
+    MODE_ADMATCH = 3,  // reference rule: k_c > active ? AS : TKO
+    MODE_HEUR = 4,     // AdMatch rule extended with an iNFA cost term
+    MODE_LEARNED = 5,  // MLP selector, MODE_HEUR as fallback
+    MODE_LOG = 6,      // run all three methods per step and log timings
+};
+
+enum Method : int32_t { M_INFA = 0, M_TKO = 1, M_AS = 2 };
+
+constexpr int kFeatures = 16;
+constexpr int kHidden = 64;
+constexpr int kOutputs = 3;
+// Packed MLP layout (floats): W1[64][16], b1[64], W2[64][64], b2[64],
+// W3[3][64], b3[3]; input normalisation folded into W1/b1.
+constexpr int kMlpW1 = 0;
+constexpr int kMlpB1 = kMlpW1 + kHidden * kFeatures;
+constexpr int kMlpW2 = kMlpB1 + kHidden;
+constexpr int kMlpB2 = kMlpW2 + kHidden * kHidden;
+constexpr int kMlpW3 = kMlpB2 + kHidden;
+constexpr int kMlpB3 = kMlpW3 + kOutputs * kHidden;
+constexpr int kMlpParams = kMlpB3 + kOutputs;  // 5,443
+constexpr int kLogRecord = kFeatures + kOutputs + 1;  // features, 3 times, chosen
+
+__device__ __forceinline__ bool word_char(uint32_t c) {
+    return (c - 'A' < 26u) || (c - 'a' < 26u) || (c - '0' < 10u) || c == '_';
+}
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+// Orders this wave's LDS accesses across lanes (one wave owns a string).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ int wave_sum(int v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_sum_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// inclusive prefix sum across the 64 lanes
+__device__ __forceinline__ int wave_scan(int v) {
+    const int l = lane_id();
+    for (int o = 1; o < 64; o <<= 1) {
+        int t = __shfl_up(v, o, 64);
+        if (l >= o) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ bool test_bit(const uint64_t* v, uint32_t i) { return (v[i >> 6] >> (i & 63)) & 1ull; }
+
+}  // namespace admatch

@@ -1,0 +1,77 @@
+// Host-side compiled rule set: the .nfa parser and the per-method tables.
+//
+// Restates the reference's TransitionGraph (src/transition_graph.h:55-104,
+// src/transition_graph.cpp:82-411) for one process-wide graph that feeds all
+// three step methods, with these MI355X-side choices:
+//   * 32-bit state ids (the reference's unsigned short caps a rule set at
+//     65,535 states, src/common.h:6);
+//   * state bit vectors in u64 words, bit i in word i/64, LSB first (the
+//     reference uses u32 words with the same bit order, state_vector.cpp:52-62);
+//   * the active-state table is sparse: one row of destination ids per
+//     (state, byte) pair instead of the dense [257][S][W] bit table
+//     (transition_graph.cpp:119-126), which would need 257*S^2/8 bytes.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace admatch {
+
+constexpr int kSymbols = 257;      // 256 byte values + word boundary (src/common.h:13)
+constexpr int kWordBoundary = 256;  // src/common.h:15
+constexpr int kMaxK = 1024;        // src/common.h:21
+constexpr int32_t kMaxStates = 1 << 26;
+
+struct Edge {
+    uint32_t src, dst;
+};
+
+struct Graph {
+    int32_t S = 0;  // states
+    int32_t W = 0;  // u64 words per state vector
+    std::vector<uint64_t> init, persis, accept;
+
+    // iNFA list: per-symbol (src,dst) sorted unique, CSR over 257 symbols
+    // (merge_transitions, transition_graph.cpp:369-411).
+    std::array<int32_t, kSymbols + 1> sym_off{};
+    std::vector<Edge> list;
+
+    // accept map: state -> rule ids, first accepting line per state wins
+    // (transition_graph.cpp:282-283).  CSR over states.
+    std::vector<int32_t> acc_off, acc_rules;
+
+    // TKO partition (transition_graph.cpp:298-357).  koff: 258 prefix sums
+    // of the top-k group count per symbol; shift[K]: offset dst-src of each
+    // group; lim[K*W]: source mask of each group; loff/left: the leftover
+    // transitions per symbol (word-boundary transitions are all leftover).
+    int32_t tko_threshold = 0;
+    std::vector<int32_t> tko_koff, tko_shift, tko_loff;
+    std::vector<uint64_t> tko_lim;
+    std::vector<Edge> tko_left;
+
+    // Sparse active-state rows: destinations of state s on byte c are
+    // as_dst[as_off[s*256+c] .. as_off[s*256+c+1]), ascending.  Persistent
+    // 0|255 self-loops are not stored (every method applies the persistent
+    // mask instead).
+    std::vector<uint32_t> as_off, as_dst;
+
+    // parse statistics
+    int64_t n_persis_lines = 0, n_bad_lines = 0, n_raw_transitions = 0;
+
+    int32_t wb_count() const { return sym_off[kWordBoundary + 1] - sym_off[kWordBoundary]; }
+    uint64_t content_hash() const;
+};
+
+// Parse a .nfa file (format of transition_graph.cpp:82-294).  Returns 0 on
+// success, otherwise a nonzero code and a message in err.
+int load_nfa(const char* path, Graph& g, std::string& err);
+
+// TKO top-k partition; threshold < 0 selects the reference's S/16.
+void build_tko(Graph& g, int32_t threshold = -1);
+
+// Sparse active-state rows from the iNFA lists.
+void build_as(Graph& g);
+
+}  // namespace admatch

@@ -1,0 +1,701 @@
+// Scan kernels for gfx950 (wave64, 160 KB LDS per CU).
+//
+//  * plain_infa_kernel: the single-kernel iNFA path -- one workgroup per
+//    string, state bit vectors in LDS, every step scatters the byte's whole
+//    transition list (the design of src/iNFA_kernel.cu:12-173, re-laid out
+//    for u64 words and LDS ds_or atomics).  It is the throughput baseline.
+//  * engine_kernel: the adaptive engine -- a persistent grid whose waves pull
+//    strings longest-first from a work queue; one wave owns one string, so a
+//    step needs no workgroup barrier.  Each step runs one of three methods,
+//    chosen per step by the selector:
+//      iNFA  scatter over the byte's (src,dst) list      (src/iNFA_kernel.cu:89-132)
+//      TKO   top-k offset groups as word shifts + leftover scatter
+//                                                        (src/TKO_kernel.cu:89-188)
+//      AS    destination rows of the active states       (src/AS_kernel.cu:74-125)
+//    The active set is kept both as a bitmap and, while it has at most LCAP
+//    states, as a list, so the AS and iNFA methods never touch all W words.
+//  * bin kernels: longest-first processing order by length bins.
+//  * MLP selector inference is fused into the engine's step loop.
+#include "device.hpp"
+
+namespace admatch {
+
+// ------------------------------------------------------------------ common
+__device__ __forceinline__ bool set_new(uint64_t* bits, uint32_t d) {
+    const unsigned long long m = 1ull << (d & 63);
+    const unsigned long long old = atomicOr(reinterpret_cast<unsigned long long*>(&bits[d >> 6]), m);
+    return !(old & m);
+}
+
+__device__ __forceinline__ void set_bit_atomic(uint64_t* bits, uint32_t d) {
+    atomicOr(reinterpret_cast<unsigned long long*>(&bits[d >> 6]), 1ull << (d & 63));
+}
+
+// Byte i of a string, read through a 256-byte window held one u32 per lane.
+struct ByteWindow {
+    const uint8_t* p;
+    int64_t len;
+    int64_t base;  // first byte index of the window
+    uint32_t w;    // lane l holds bytes base+4l .. base+4l+3
+
+    __device__ void load(int64_t b) {
+        base = b;
+        const int64_t i = b + 4 * lane_id();
+        uint32_t v = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (i + k < len) v |= uint32_t(p[i + k]) << (8 * k);
+        w = v;
+    }
+    __device__ uint32_t at(int64_t i) {  // i uniform, base <= i
+        if (i - base >= 256) load(i);
+        const int off = int(i - base);
+        const uint32_t v = uint32_t(__builtin_amdgcn_readlane(int(w), off >> 2));
+        return (v >> (8 * (off & 3))) & 0xffu;
+    }
+};
+
+// Appends v for every lane with pred; returns the new (uniform) count,
+// which may exceed cap (the list is then incomplete).
+template <int CAP>
+__device__ __forceinline__ int list_append(uint32_t* list, int n, bool pred, uint32_t v) {
+    const uint64_t m = __ballot(pred);
+    if (pred) {
+        const int idx = n + int(mbcnt(m));
+        if (idx < CAP) list[idx] = v;
+    }
+    return n + __popcll(m);
+}
+
+// Ascending ids of the set bits of bits[0..W) into list; returns the count.
+template <int CAP>
+__device__ int rebuild_list(const uint64_t* bits, int W, uint32_t* list) {
+    int n = 0;
+    for (int j0 = 0; j0 < W; j0 += 64) {
+        const int j = j0 + lane_id();
+        uint64_t v = j < W ? bits[j] : 0ull;
+        const int cnt = __popcll(v);
+        const int incl = wave_scan(cnt);
+        const int tot = uniform(__shfl(incl, 63, 64));
+        if (n + tot <= CAP) {
+            int pos = n + incl - cnt;
+            while (v) {
+                const int b = __ffsll(static_cast<unsigned long long>(v)) - 1;
+                v &= v - 1;
+                list[pos++] = uint32_t(j * 64 + b);
+            }
+        }
+        n += tot;
+    }
+    return n;
+}
+
+// Writes the accepted states (cur & accept, ascending) of string sid.
+__device__ void emit_accepts(const DevGraph& g, const ScanArgs& a, int sid, const uint64_t* cur) {
+    int total = 0;
+    for (int j0 = 0; j0 < g.W; j0 += 64) {
+        const int j = j0 + lane_id();
+        const uint64_t v = j < g.W ? (cur[j] & g.accept[j]) : 0ull;
+        total += wave_sum(__popcll(v));
+    }
+    total = uniform(total);
+    int64_t start = 0;
+    if (total > 0) {
+        unsigned long long s0 = 0;
+        if (lane_id() == 0) s0 = atomicAdd(a.pool_top, (unsigned long long)total);
+        start = int64_t(__shfl(static_cast<long long>(s0), 0, 64));
+        if (start + total > a.pool_cap) {
+            if (lane_id() == 0) atomicOr(a.overflow, 1);
+            total = -total;  // marks a truncated result
+        } else {
+            int n = 0;
+            for (int j0 = 0; j0 < g.W; j0 += 64) {
+                const int j = j0 + lane_id();
+                uint64_t v = j < g.W ? (cur[j] & g.accept[j]) : 0ull;
+                const int cnt = __popcll(v);
+                const int incl = wave_scan(cnt);
+                int64_t pos = start + n + incl - cnt;
+                while (v) {
+                    const int b = __ffsll(static_cast<unsigned long long>(v)) - 1;
+                    v &= v - 1;
+                    a.res_pool[pos++] = uint32_t(j * 64 + b);
+                }
+                n += uniform(__shfl(incl, 63, 64));
+            }
+        }
+    }
+    if (lane_id() == 0) {
+        a.res_count[sid] = total;
+        a.res_start[sid] = start;
+    }
+}
+
+// ------------------------------------------------------------ plain iNFA
+// One workgroup of BS threads per string; cur/fut/persistent vectors and the
+// 258 symbol offsets in LDS (src/iNFA_kernel.cu:22-52).
+template <int BS>
+__global__ __launch_bounds__(BS) void plain_infa_kernel(DevGraph g, ScanArgs a) {
+    extern __shared__ uint64_t lds[];
+    const int sid = blockIdx.x;
+    const int W = g.W;
+    uint64_t* cur = lds;
+    uint64_t* fut = lds + W;
+    uint64_t* per = lds + 2 * W;
+    int32_t* soff = reinterpret_cast<int32_t*>(lds + 3 * W);
+    const int tid = threadIdx.x;
+    for (int i = tid; i < W; i += BS) {
+        cur[i] = g.init[i];
+        per[i] = g.persis[i];
+    }
+    for (int i = tid; i <= 257; i += BS) soff[i] = g.sym_off[i];
+    __syncthreads();
+    const int64_t beg = a.offsets[sid];
+    const int64_t L = a.offsets[sid + 1] - beg;
+    const uint8_t* x = a.text + beg;
+    const int wb_b = soff[256], wb_n = soff[257] - soff[256];
+    auto wb_pass = [&](uint64_t* v) {
+        // in-place pass in (src,dst) order on one thread: the
+        // thread_count == 1 semantics of src/iNFA_kernel.cu:145-164
+        if (tid == 0)
+            for (int t = 0; t < wb_n; ++t) {
+                const uint2 e = g.list[wb_b + t];
+                if (test_bit(v, e.x)) v[e.y >> 6] |= 1ull << (e.y & 63);
+            }
+        __syncthreads();
+    };
+    if (wb_n > 0 && L > 0 && word_char(x[0])) wb_pass(cur);
+    for (int64_t b = 0; b < L; ++b) {
+        for (int i = tid; i < W; i += BS) fut[i] = cur[i] & per[i];
+        __syncthreads();
+        const int c = x[b];
+        const int t0 = soff[c], tn = soff[c + 1] - soff[c];
+        for (int t = tid; t < tn; t += BS) {
+            const uint2 e = g.list[t0 + t];
+            if (test_bit(cur, e.x)) set_bit_atomic(fut, e.y);
+        }
+        uint64_t* tmp = cur;
+        cur = fut;
+        fut = tmp;
+        __syncthreads();
+        if (wb_n == 0) continue;
+        if ((b < L - 1 && word_char(x[b]) == word_char(x[b + 1])) || (b == L - 1 && !word_char(x[L - 1]))) continue;
+        wb_pass(cur);
+    }
+    // accept extraction by the first wave
+    if (tid < 64) emit_accepts(g, a, sid, cur);
+}
+
+// ------------------------------------------------------------ engine steps
+// Each helper is executed by one whole wave; counts are wave-uniform.
+
+// fut |= cur & persistent, via the active list (fut empty on entry).
+template <int CAP>
+__device__ int persist_sparse(const DevGraph& g, const uint32_t* cl, int n_cur, uint64_t* fut, uint32_t* fl) {
+    int n = 0;
+    for (int k0 = 0; k0 < n_cur; k0 += 64) {
+        const int k = k0 + lane_id();
+        const bool ok = k < n_cur;
+        const uint32_t s = ok ? cl[k] : 0u;
+        const bool p = ok && test_bit(g.persis, s);
+        if (p) set_bit_atomic(fut, s);
+        n = list_append<CAP>(fl, n, p, s);
+    }
+    return n;
+}
+
+// Scatter step over (src,dst) edges: fut |= {dst : src in cur}.
+template <int CAP, bool TRACK>
+__device__ int scatter(const uint2* edges, int cnt, const uint64_t* cur, uint64_t* fut, uint32_t* fl, int n) {
+    constexpr int U = 4;
+    for (int t0 = 0; t0 < cnt; t0 += 64 * U) {
+        uint2 e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int t = t0 + u * 64 + lane_id();
+            e[u] = t < cnt ? edges[t] : make_uint2(0xffffffffu, 0u);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool hit = e[u].x != 0xffffffffu && test_bit(cur, e[u].x);
+            if (TRACK) {
+                const bool nw = hit && set_new(fut, e[u].y);
+                n = list_append<CAP>(fl, n, nw, e[u].y);
+            } else if (hit) {
+                set_bit_atomic(fut, e[u].y);
+            }
+        }
+    }
+    return n;
+}
+
+// TKO shift groups of symbol c: fut[j] |= shift(lim_i & cur, d_i).  Lane j
+// owns word j, so no atomics.  Shifts are split as d = 64q + r with
+// 0 <= r < 64 (floor division), and the r == 0 case never shifts by 64.
+__device__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64_t* fut) {
+    const int W = g.W;
+    const int kb = g.tko_koff[c], ke = g.tko_koff[c + 1];
+    for (int i = kb; i < ke; ++i) {
+        const int d = g.tko_shift[i];
+        const int q = d >> 6;  // arithmetic shift: floor(d / 64)
+        const int r = d & 63;
+        const uint64_t* lim = g.tko_lim + int64_t(i) * W;
+        for (int j = lane_id(); j < W; j += 64) {
+            const int s0 = j - q;
+            const int s1 = s0 - 1;
+            uint64_t v = 0;
+            if (s0 >= 0 && s0 < W) v = (lim[s0] & cur[s0]) << r;
+            if (r != 0 && s1 >= 0 && s1 < W) v |= (lim[s1] & cur[s1]) >> (64 - r);
+            if (v) fut[j] |= v;
+        }
+    }
+}
+
+// AS rows of the active states on byte c, flattened so that each lane
+// fetches one destination per iteration (coalesced within a row).
+template <int CAP>
+__device__ int as_rows(const DevGraph& g, int c, const uint32_t* cl, int n_cur, uint64_t* fut, uint32_t* fl, int n) {
+    for (int k0 = 0; k0 < n_cur; k0 += 64) {
+        const int k = k0 + lane_id();
+        uint32_t rb = 0, re = 0;
+        if (k < n_cur) {
+            const uint32_t* o = g.as_off + size_t(cl[k]) * 256 + c;
+            rb = o[0];
+            re = o[1];
+        }
+        const int len = int(re - rb);
+        const int incl = wave_scan(len);
+        const int excl = incl - len;
+        const int tot = uniform(__shfl(incl, 63, 64));
+        for (int t0 = 0; t0 < tot; t0 += 64) {
+            const int t = t0 + lane_id();
+            // owner lane: smallest r with incl[r] > t
+            int lo = 0, hi = 63;
+#pragma unroll
+            for (int it = 0; it < 6; ++it) {
+                const int mid = (lo + hi) >> 1;
+                const int v = __shfl(incl, mid, 64);
+                if (v > t) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint32_t rbo = uint32_t(__shfl(int(rb), lo, 64));
+            const int exo = __shfl(excl, lo, 64);
+            const bool ok = t < tot;
+            const uint32_t d = ok ? g.as_dst[rbo + uint32_t(t - exo)] : 0u;
+            const bool nw = ok && set_new(fut, d);
+            n = list_append<CAP>(fl, n, nw, d);
+        }
+    }
+    return n;
+}
+
+// ------------------------------------------------------------ selector
+struct StepCtx {
+    int c, n_act, sparse, T, K, Lft, last_m;
+    int64_t pos, len;
+    float last_cycles;
+};
+
+__device__ __forceinline__ int heuristic_choice(const DevGraph& g, const StepCtx& s) {
+    // AdMatch rule (src/AdMatch_kernel.cu:87) extended with an iNFA cost term
+    if (s.sparse && s.K > s.n_act) return M_AS;
+    const int64_t tko_cost = int64_t(s.K) * g.W + s.Lft;
+    return int64_t(s.T) <= tko_cost ? M_INFA : M_TKO;
+}
+
+__device__ __forceinline__ void make_features(const DevGraph& g, const StepCtx& s, float* f) {
+    f[0] = __log2f(1.f + float(s.n_act));
+    f[1] = 100.f * float(s.n_act) / float(g.S);
+    f[2] = __log2f(1.f + float(s.T));
+    f[3] = float(s.K);
+    f[4] = __log2f(1.f + float(s.Lft));
+    f[5] = __log2f(float(g.W));
+    f[6] = word_char(uint32_t(s.c)) ? 1.f : 0.f;
+    f[7] = float(s.c >> 5);
+    f[8] = float(s.pos) / float(s.len > 0 ? s.len : 1);
+    f[9] = __log2f(1.f + float(s.len));
+    f[10] = float(s.sparse);
+    f[11] = s.last_m == M_INFA ? 1.f : 0.f;
+    f[12] = s.last_m == M_TKO ? 1.f : 0.f;
+    f[13] = s.last_m == M_AS ? 1.f : 0.f;
+    f[14] = __log2f(1.f + s.last_cycles);
+    f[15] = __log2f(1.f + float(s.K) * float(g.W));
+}
+
+// 16 -> 64 -> 64 -> 3 MLP evaluated by one wave: lane j is hidden unit j.
+// W1 is stored input-major [16][64], W2 as [64 in][64 out], so every
+// weight load is one coalesced 256-byte row.  Returns argmin of the three
+// predicted log step costs among the allowed methods.
+__device__ int mlp_choice(const float* w, const float* f, bool allow_as) {
+    const int j = lane_id();
+    float h = w[kMlpB1 + j];
+#pragma unroll
+    for (int i = 0; i < kFeatures; ++i) h += w[kMlpW1 + i * kHidden + j] * f[i];
+    h = fmaxf(h, 0.f);
+    float h2 = w[kMlpB2 + j];
+#pragma unroll 8
+    for (int k = 0; k < kHidden; ++k) {
+        const float hk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h), k));
+        h2 += w[kMlpW2 + k * kHidden + j] * hk;
+    }
+    h2 = fmaxf(h2, 0.f);
+    float best = 3.0e38f;
+    int arg = M_INFA;
+#pragma unroll
+    for (int m = 0; m < kOutputs; ++m) {
+        const float o = wave_sum_f(w[kMlpW3 + m * kHidden + j] * h2) + w[kMlpB3 + m];
+        const float ou = __int_as_float(uniform(__float_as_int(o)));
+        if ((m != M_AS || allow_as) && ou < best) {
+            best = ou;
+            arg = m;
+        }
+    }
+    return arg;
+}
+
+// ------------------------------------------------------------ engine
+template <int CAP>
+struct Lane {
+    static constexpr size_t words(int W) { return size_t(2) * W + CAP; }  // u64 words per wave
+};
+
+// Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
+// Returns the fut count (list valid iff *fut_sparse).
+template <int CAP>
+__device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, const uint32_t* cl, int n_cur,
+                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse) {
+    int n = 0;
+    const int W = g.W;
+    if (m == M_TKO) {
+        tko_groups(g, c, cur, fut);
+        wave_sync();
+        scatter<CAP, false>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, fl, 0);
+        *fut_sparse = false;
+        return -1;
+    }
+    if (sparse) {
+        n = persist_sparse<CAP>(g, cl, n_cur, fut, fl);
+    } else {
+        for (int j = lane_id(); j < W; j += 64) fut[j] = cur[j] & g.persis[j];
+    }
+    wave_sync();
+    if (m == M_AS && sparse) {
+        n = as_rows<CAP>(g, c, cl, n_cur, fut, fl, n);
+    } else if (sparse) {
+        n = scatter<CAP, true>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, n);
+    } else {
+        scatter<CAP, false>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, 0);
+        n = -1;
+    }
+    *fut_sparse = sparse && n <= CAP;
+    return n;
+}
+
+template <int CAP>
+__device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list, int n, bool sparse, int W) {
+    if (sparse) {
+        for (int k = lane_id(); k < n; k += 64) bits[list[k] >> 6] = 0ull;
+    } else {
+        for (int j = lane_id(); j < W; j += 64) bits[j] = 0ull;
+    }
+}
+
+template <int CAP>
+__device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
+                            uint32_t* listA, uint32_t* listB, int* msteps) {
+    const int W = g.W;
+    const int64_t beg = a.offsets[sid];
+    const int64_t L = a.offsets[sid + 1] - beg;
+    uint64_t* cur = bitsA;
+    uint64_t* fut = bitsB;
+    uint32_t* cl = listA;
+    uint32_t* fl = listB;
+    // invariant: both bitmaps are zero here
+    for (int j = lane_id(); j < W; j += 64) cur[j] = g.init[j];
+    wave_sync();
+    int n_act = uniform(rebuild_list<CAP>(cur, W, cl));
+    bool sparse = n_act <= CAP;
+    ByteWindow win{a.text + beg, L, 0, 0u};
+    if (L > 0) win.load(0);
+    const bool has_wb = g.wb_count > 0;
+
+    auto wb_pass = [&]() {
+        // in place, list order, one lane (thread_count == 1 semantics of
+        // src/iNFA_kernel.cu:145-164)
+        int n = n_act;
+        if (lane_id() == 0) {
+            for (int t = 0; t < g.wb_count; ++t) {
+                const uint2 e = g.list[g.wb_begin + t];
+                if (test_bit(cur, e.x)) {
+                    const uint64_t m = 1ull << (e.y & 63);
+                    if (!(cur[e.y >> 6] & m)) {
+                        cur[e.y >> 6] |= m;
+                        if (sparse && n < CAP) cl[n] = e.y;
+                        ++n;
+                    }
+                }
+            }
+        }
+        n_act = uniform(__builtin_amdgcn_readlane(n, 0));
+        if (n_act > CAP) sparse = false;
+        wave_sync();
+    };
+    if (has_wb && L > 0 && word_char(win.at(0))) wb_pass();
+
+    StepCtx ctx;
+    ctx.len = L;
+    ctx.last_m = M_INFA;
+    ctx.last_cycles = 0.f;
+    int cached = -1;
+    for (int64_t b = 0; b < L; ++b) {
+        const int c = int(win.at(b));
+        ctx.c = c;
+        ctx.n_act = n_act;
+        ctx.sparse = sparse;
+        ctx.pos = b;
+        const int4 st = g.sym_stats[c];
+        ctx.T = st.x;
+        ctx.K = st.y;
+        ctx.Lft = st.z;
+        int m;
+        bool log_step = false;
+        switch (a.mode) {
+            case MODE_INFA: m = M_INFA; break;
+            case MODE_TKO: m = M_TKO; break;
+            case MODE_AS: m = sparse ? M_AS : M_INFA; break;
+            case MODE_ADMATCH: m = (sparse && ctx.K > n_act) ? M_AS : M_TKO; break;
+            case MODE_LEARNED:
+                if (a.mlp != nullptr) {
+                    if (cached < 0 || (b % a.mlp_every) == 0) {
+                        float f[kFeatures];
+                        make_features(g, ctx, f);
+                        cached = mlp_choice(a.mlp, f, true);
+                    }
+                    m = cached;
+                    if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
+                } else {
+                    m = heuristic_choice(g, ctx);
+                }
+                break;
+            case MODE_LOG:
+                m = heuristic_choice(g, ctx);
+                log_step = ((uint64_t(sid) * 2654435761ull + uint64_t(b)) % uint64_t(a.log_stride)) == 0;
+                break;
+            default: m = heuristic_choice(g, ctx); break;
+        }
+        bool fsp;
+        int n_fut;
+        if (!log_step) {
+            n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp);
+        } else {
+            // run every method on the same state and time each; keep the
+            // iNFA result (all three are the same set)
+            float tm[3] = {-1.f, -1.f, -1.f};
+            const int order[3] = {M_TKO, M_AS, M_INFA};
+            n_fut = 0;
+            fsp = false;
+            for (int oi = 0; oi < 3; ++oi) {
+                const int mm = order[oi];
+                if (mm == M_AS && !sparse) continue;
+                __builtin_amdgcn_s_waitcnt(0);
+                const uint64_t t0 = clock64();
+                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp);
+                wave_sync();
+                __builtin_amdgcn_s_waitcnt(0);
+                const uint64_t t1 = clock64();
+                tm[mm] = float(t1 - t0);
+                if (oi < 2) {
+                    clear_bits<CAP>(fut, fl, n_fut, fsp, W);
+                    wave_sync();
+                }
+            }
+            float f[kFeatures];
+            make_features(g, ctx, f);
+            unsigned long long slot = 0;
+            if (lane_id() == 0) slot = atomicAdd(a.log_top, 1ull);
+            slot = __shfl(static_cast<long long>(slot), 0, 64);
+            if (int64_t(slot) < a.log_cap) {
+                float* rec = a.log_buf + int64_t(slot) * kLogRecord;
+                const int l = lane_id();
+                if (l < kFeatures) rec[l] = f[l];
+                if (l < 3) rec[kFeatures + l] = tm[l];
+                if (l == 0) rec[kFeatures + 3] = float(m);
+            }
+            m = M_INFA;
+        }
+        wave_sync();
+        clear_bits<CAP>(cur, cl, n_act, sparse, W);
+        // swap
+        uint64_t* tb = cur;
+        cur = fut;
+        fut = tb;
+        uint32_t* tl = cl;
+        cl = fl;
+        fl = tl;
+        wave_sync();
+        if (fsp) {
+            n_act = n_fut;
+            sparse = true;
+        } else {
+            n_act = uniform(rebuild_list<CAP>(cur, W, cl));
+            sparse = n_act <= CAP;
+        }
+        msteps[m] += 1;
+        ctx.last_m = m;
+        if (has_wb) {
+            const uint32_t xb = uint32_t(c);
+            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(win.at(b + 1))) : word_char(xb);
+            // win.at(b+1) may have moved the window forward: b is not read again
+            if (boundary) wb_pass();
+        }
+    }
+    wave_sync();
+    emit_accepts(g, a, sid, cur);
+    wave_sync();
+    clear_bits<CAP>(cur, cl, n_act, sparse, W);
+    wave_sync();
+}
+
+template <int WPB, int CAP>
+__global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a) {
+    extern __shared__ uint64_t lds[];
+    const int wave = threadIdx.x >> 6;
+    const size_t per = Lane<CAP>::words(g.W);
+    uint64_t* base = lds + per * wave;
+    uint64_t* bitsA = base;
+    uint64_t* bitsB = base + g.W;
+    uint32_t* listA = reinterpret_cast<uint32_t*>(base + 2 * g.W);
+    uint32_t* listB = listA + CAP;
+    for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
+    wave_sync();
+    int msteps[3] = {0, 0, 0};  // steps per method, this wave
+    for (;;) {
+        int idx = 0;
+        if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
+        idx = uniform(__builtin_amdgcn_readlane(idx, 0));
+        if (idx >= a.n) break;
+        const int sid = a.order ? a.order[idx] : idx;
+        scan_string<CAP>(g, a, sid, bitsA, bitsB, listA, listB, msteps);
+    }
+    if (a.method_steps != nullptr && lane_id() == 0)
+        for (int m = 0; m < 3; ++m)
+            if (msteps[m]) atomicAdd(&a.method_steps[m], (unsigned long long)msteps[m]);
+}
+
+// ------------------------------------------------------------ ordering
+constexpr int kBins = 40;
+
+__device__ __forceinline__ int len_bin(int64_t len) {
+    // descending length bins: bin 0 holds the longest strings
+    const int lg = len <= 0 ? 0 : 64 - __clzll(static_cast<unsigned long long>(len));  // 1..40
+    return kBins - 1 - min(lg, kBins - 1);
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = v;
+}
+
+// counters layout (int32): [0] queue, [1] overflow, [2..3] pool_top (u64),
+// [4..4+kBins) bin counts, [4+kBins..4+2kBins) bin cursors
+__global__ void bin_count_kernel(const int64_t* offsets, int32_t n, int32_t* bins) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicAdd(&bins[len_bin(offsets[i + 1] - offsets[i])], 1);
+}
+
+__global__ void bin_scatter_kernel(const int64_t* offsets, int32_t n, const int32_t* bins, int32_t* cursors,
+                                   int32_t* order) {
+    __shared__ int32_t start[kBins];
+    if (threadIdx.x == 0) {
+        int32_t s = 0;
+        for (int b = 0; b < kBins; ++b) {
+            start[b] = s;
+            s += bins[b];
+        }
+    }
+    __syncthreads();
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int b = len_bin(offsets[i + 1] - offsets[i]);
+        order[start[b] + atomicAdd(&cursors[b], 1)] = i;
+    }
+}
+
+// ------------------------------------------------------------ launchers
+namespace {
+template <int WPB, int CAP>
+hipError_t engine_launch_t(int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((engine_kernel<WPB, CAP>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
+    return hipGetLastError();
+}
+template <int WPB, int CAP>
+hipError_t engine_occ_t(size_t lds, int* blocks) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, engine_kernel<WPB, CAP>, 64 * WPB, lds);
+}
+template <int BS>
+hipError_t plain_launch_t(int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&plain_infa_kernel<BS>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr = true;
+    }
+    hipLaunchKernelGGL((plain_infa_kernel<BS>), dim3(n), dim3(BS), lds, st, g, a);
+    return hipGetLastError();
+}
+}  // namespace
+
+#define ADM_DISPATCH_ENGINE(FN, ...)                                   \
+    switch (wpb * 10000 + cap) {                                       \
+        case 10256: return FN<1, 256>(__VA_ARGS__);                    \
+        case 20256: return FN<2, 256>(__VA_ARGS__);                    \
+        case 40256: return FN<4, 256>(__VA_ARGS__);                    \
+        case 10512: return FN<1, 512>(__VA_ARGS__);                    \
+        case 20512: return FN<2, 512>(__VA_ARGS__);                    \
+        case 40512: return FN<4, 512>(__VA_ARGS__);                    \
+        case 11024: return FN<1, 1024>(__VA_ARGS__);                   \
+        case 21024: return FN<2, 1024>(__VA_ARGS__);                   \
+        case 41024: return FN<4, 1024>(__VA_ARGS__);                   \
+        default: return hipErrorInvalidValue;                          \
+    }
+
+hipError_t launch_engine(int wpb, int cap, int grid, size_t lds, hipStream_t st, const DevGraph& g,
+                         const ScanArgs& a) {
+    ADM_DISPATCH_ENGINE(engine_launch_t, grid, lds, st, g, a)
+}
+
+hipError_t engine_occupancy(int wpb, int cap, size_t lds, int* blocks) {
+    ADM_DISPATCH_ENGINE(engine_occ_t, lds, blocks)
+}
+
+hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
+    switch (bs) {
+        case 64: return plain_launch_t<64>(n, lds, st, g, a);
+        case 128: return plain_launch_t<128>(n, lds, st, g, a);
+        case 256: return plain_launch_t<256>(n, lds, st, g, a);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_fill(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
+    const int grid = int(n < 65536 ? (n + 255) / 256 : 256);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(grid > 0 ? grid : 1), dim3(256), 0, st, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_order(const int64_t* offsets, int32_t n, int32_t* ctr_bins, int32_t* ctr_cursors, int32_t* order,
+                        hipStream_t st) {
+    const int grid = (n + 255) / 256;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(bin_count_kernel, dim3(grid), dim3(256), 0, st, offsets, n, ctr_bins);
+    hipLaunchKernelGGL(bin_scatter_kernel, dim3(grid), dim3(256), 0, st, offsets, n, ctr_bins, ctr_cursors, order);
+    return hipGetLastError();
+}
+
+}  // namespace admatch

@@ -1,0 +1,197 @@
+"""Benchmark of the batched scan path on one MI355X.
+
+A "step" is one BatchedScan of the whole synthetic batch, inputs already
+resident in HBM: the engine kernel (plus its two small length-binning
+kernels) replayed from a hipGraph, then the accepted states copied back and
+mapped to sorted rule ids on the host.  The default workload is the medium
+configuration: 1,000 seed-generated rules (~22.8k NFA states) against
+16,384 HTTP-like strings with lognormal lengths (median 600 B, max 8 KB).
+
+Printed: ONE JSON line with input GB/s of the headline selector, the same
+for the plain single-kernel iNFA path and the heuristic selectors, per
+request latency percentiles, the roofline of the dominant kernel (HBM
+bound: 1 input byte + 8 descriptor bytes per string, over the kernel's
+average duration from hipEvents on the engine stream) and the CPU oracle's
+single-core rate on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SELECTOR_PATH = os.path.join(ROOT, "admatch_amd", "selector", "selector_weights.npy")
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_mode(s, E, d_buf, d_off, n, kind, mode, steps, warmup):
+    """Returns (ms per step list, kernel ms list, results of the last step)."""
+    res = None
+    for _ in range(warmup):
+        s.scan_device(d_buf.ptr, d_off.ptr, n, kind, mode, use_graph=True)
+        res = s.collect(n)
+    step_ms, kern_ms = [], []
+    E.Hip.rt().hipDeviceSynchronize()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        s.scan_device(d_buf.ptr, d_off.ptr, n, kind, mode, use_graph=True)
+        res = s.collect(n)  # synchronises the engine stream
+        step_ms.append((time.perf_counter() - t0) * 1e3)
+        kern_ms.append(s.last_stats()[0])
+    E.Hip.rt().hipDeviceSynchronize()
+    return step_ms, kern_ms, res
+
+
+def cpu_baseline(path, inputs, budget_s=12.0):
+    """Single-threaded oracle on a bounded prefix of the same batch."""
+    import oracle
+
+    o = oracle.OracleNFA(path)
+    buf, off = inputs.buf, inputs.offsets
+    n = off.size - 1
+    # calibrate on a few strings, then size the sample to ~budget_s
+    k = min(16, n)
+    t0 = time.perf_counter()
+    o.batch_scan(buf, off[: k + 1], nthreads=1)
+    dt = max(time.perf_counter() - t0, 1e-6)
+    rate = (off[k] - off[0]) / dt
+    want = int(min(n, max(k, budget_s * rate / max(1.0, (off[-1] - off[0]) / n))))
+    sub = off[: want + 1]
+    t0 = time.perf_counter()
+    o.batch_scan(buf, sub, nthreads=1)
+    dt = time.perf_counter() - t0
+    nbytes = int(sub[-1] - sub[0])
+    return {"value": round(nbytes / dt / 1e9, 6), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"first {want} of {n} strings ({nbytes} B), single-threaded oracle, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="medium", choices=["small", "medium", "large"])
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--modes", default="plain,admatch,heuristic,as,learned")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "strings shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from admatch_amd import engine as E
+    from admatch_amd import synth
+
+    t0 = time.time()
+    w = synth.make_workload(args.config, n=args.n)
+    n = w.inputs.n
+    total_bytes = int(w.inputs.offsets[-1])
+    log(f"workload {args.config}: {w.ruleset.state_count} states, {n} strings, {total_bytes} B "
+        f"({time.time() - t0:.1f} s to generate)")
+    tmp = tempfile.mkdtemp()
+    path = os.path.join(tmp, f"{args.config}.nfa")
+    w.ruleset.write(path)
+
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    info = s.info()
+    tuned = None
+    tune_path = os.path.join(ROOT, "admatch_amd", "selector", "autotune.json")
+    if os.path.exists(tune_path):
+        with open(tune_path) as f:
+            tuned = json.load(f).get(str(info["hash"]))
+    if tuned:
+        s.set_variant(tuned["wpb"], tuned["cap"], tuned.get("plain_bs", 64))
+    learned_ok = False
+    if os.path.exists(SELECTOR_PATH):
+        s.set_selector(np.load(SELECTOR_PATH), every=8)
+        learned_ok = True
+    d_buf, d_off = E.DeviceArray(w.inputs.buf), E.DeviceArray(w.inputs.offsets)
+
+    modes = {
+        "plain": (E.KIND_PLAIN, E.MODE_INFA),
+        "admatch": (E.KIND_ENGINE, E.MODE_ADMATCH),
+        "heuristic": (E.KIND_ENGINE, E.MODE_HEURISTIC),
+        "as": (E.KIND_ENGINE, E.MODE_AS),
+        "learned": (E.KIND_ENGINE, E.MODE_LEARNED),
+        "infa_engine": (E.KIND_ENGINE, E.MODE_INFA),
+        "tko": (E.KIND_ENGINE, E.MODE_TKO),
+    }
+    results, ref = {}, None
+    for name in args.modes.split(","):
+        kind, mode = modes[name]
+        steps = args.steps if name in ("learned",) else max(2, min(args.steps, 5))
+        log(f"mode {name}: warmup {args.warmup}, steps {steps}")
+        step_ms, kern_ms, res = time_mode(s, E, d_buf, d_off, n, kind, mode, steps, args.warmup)
+        out = E.unpack(*res)
+        if ref is None:
+            ref = out
+        elif out != ref:
+            raise SystemExit(f"mode {name} disagrees with {list(results)[0]}")
+        _, msteps = s.last_stats()
+        ms = float(np.mean(step_ms))
+        results[name] = {
+            "ms_per_step": ms, "gbs": total_bytes / (ms * 1e-3) / 1e9,
+            "kernel_ms": float(np.mean(kern_ms)), "p50_step_ms": float(np.percentile(step_ms, 50)),
+            "p99_step_ms": float(np.percentile(step_ms, 99)), "method_steps": msteps, "steps": steps,
+        }
+        log(f"  {name}: {ms:.3f} ms/step, {results[name]['gbs']:.3f} GB/s, kernel {results[name]['kernel_ms']:.3f} ms, "
+            f"method steps {msteps}")
+
+    head = "learned" if "learned" in results else list(results)[-1]
+    hr = results[head]
+    nbytes_alg = total_bytes + 8 * n
+    achieved = nbytes_alg / (hr["kernel_ms"] * 1e-3) / 1e9
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"engine_kernel_{args.config}_pmc.json")
+    if os.path.exists(pmc):
+        with open(pmc) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    line = {
+        "metric": "input GB/s scanned (learned-selector BatchedScan), p50/p99 us/request",
+        "value": round(hr["gbs"], 4),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": hr["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": round(hr["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (seeded rules and HTTP-like strings)",
+        "config": {"workload": f"{args.config}: {w.ruleset.state_count} states from "
+                               f"{len(w.ruleset.rules)} rules, {n} strings, {total_bytes} B",
+                   "variant": dict(zip(("wpb", "cap", "plain_bs"), s.variant())),
+                   "learned_selector": learned_ok},
+        "modes": {k: {"gbs": round(v["gbs"], 4), "ms_per_step": round(v["ms_per_step"], 4),
+                      "kernel_ms": round(v["kernel_ms"], 4), "method_steps": v["method_steps"]}
+                  for k, v in results.items()},
+        "speedup_vs_plain_infa": round(hr["gbs"] / results["plain"]["gbs"], 3) if "plain" in results else None,
+        "us_per_request": round(hr["ms_per_step"] * 1e3 / n, 4),
+        "p50_us_per_request": round(hr["p50_step_ms"] * 1e3 / n, 4),
+        "p99_us_per_request": round(hr["p99_step_ms"] * 1e3 / n, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic},
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle, 1 thread)")
+        line["cpu_baseline"] = cpu_baseline(path, w.inputs)
+    s.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,146 @@
+/*
+ * admatch.h -- C ABI of the MI355X scan engine.
+ *
+ * Every entry point is plain C: opaque handle, plain pointers and sizes,
+ * int status codes (0 = success, negative = error; ita_last_error() gives the
+ * message of the calling thread's last failure).  The C++ surface of the
+ * reference, include/ITA_regex.h, is implemented on top of these calls
+ * (see ITA_regex.h in this directory); FFI bindings (ctypes, cgo, JNI) bind
+ * these symbols directly.
+ *
+ * Handles are independent: each owns its compiled rule set, device tables,
+ * HIP stream and buffers, so calls on distinct handles may run on distinct
+ * threads.  Calls on one handle must not overlap.
+ */
+#ifndef ADMATCH_H
+#define ADMATCH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define ITA_OK 0
+#define ITA_ERR_ARG (-1)
+#define ITA_ERR_LOAD (-2)
+#define ITA_ERR_HIP (-3)
+#define ITA_ERR_STATE (-4)
+#define ITA_ERR_CAPACITY (-5)
+#define ITA_ERR_TOO_LARGE (-6)
+#define ITA_ERR_BUDGET (-7)
+
+/* Flag bits; values 1..32 are those of the reference, include/ITA_regex.h:10-24. */
+#define ITA_INFA_KERNEL 1
+#define ITA_AS_KERNEL (1 << 1)
+#define ITA_TKO_KERNEL (1 << 2)
+#define ITA_PROFILER_MODE (1 << 3)
+#define ITA_SHOW_RESULTS (1 << 4)
+#define ITA_AD_MATCHER (1 << 5)
+/* New bits (no counterpart in the reference). */
+#define ITA_LEARNED_SELECTOR (1 << 6) /* with AD_MATCHER: MLP step selector */
+#define ITA_HEURISTIC_SELECTOR (1 << 7) /* with AD_MATCHER: AdMatch rule + iNFA cost term */
+
+/* Step-selector modes of the engine kernel (ita_scan_device's mode). */
+#define ITA_MODE_INFA 0
+#define ITA_MODE_TKO 1
+#define ITA_MODE_AS 2
+#define ITA_MODE_ADMATCH 3
+#define ITA_MODE_HEURISTIC 4
+#define ITA_MODE_LEARNED 5
+#define ITA_MODE_LOG 6
+/* ita_scan_device's kind: the plain single-kernel iNFA path, or the engine. */
+#define ITA_KIND_PLAIN 0
+#define ITA_KIND_ENGINE 1
+
+typedef struct ita_handle ita_handle;
+
+/* Parse a .nfa file and build the per-method tables on the host.
+ * Replaces the constructors ita_scratch::ita_scratch and
+ * ad_scratch::ad_scratch (reference src/Interface.cu:20-40, graph build
+ * src/transition_graph.cpp:82-365).  flags: exactly one of INFA/AS/TKO
+ * kernel bits, or AD_MATCHER (reference FLAG_VERIFICATION, Interface.cu:12-18);
+ * otherwise ITA_ERR_ARG.  A load failure returns ITA_ERR_LOAD (the reference
+ * calls exit(-1), Interface.cu:28-31). */
+int ita_create(uint16_t flags, const char* nfa_path, ita_handle** out);
+
+/* Upload the tables to the GPU (reference allocScratch, Interface.cu:42-116). */
+int ita_alloc(ita_handle* h);
+
+/* Release device memory (reference freeScratch, Interface.cu:124-142). */
+int ita_free(ita_handle* h);
+
+/* Destroy the handle (host tables too; the reference leaks them). */
+int ita_destroy(ita_handle* h);
+
+/* Scan one NUL-terminated string, length strlen(text) (reference Scan,
+ * Interface.cu:150-172).  Writes up to cap sorted unique rule ids to out_ids
+ * and the full count to *n_out; ITA_ERR_CAPACITY if it exceeds cap. */
+int ita_scan(ita_handle* h, const char* text, int32_t* out_ids, int32_t cap, int32_t* n_out);
+
+/* Scan n strings from host memory (reference BatchedScan, Interface.cu:174-198,
+ * driver run_iNFA/run_TKO/run_AS/run_AD, e.g. src/iNFA_kernel.cu:175-343).
+ * Rule ids of string i: out_ids[out_offsets[i] .. out_offsets[i+1]), sorted
+ * unique.  out_offsets holds n+1 entries; out_ids holds cap entries.  If
+ * the total exceeds cap, out_offsets is still filled and ITA_ERR_CAPACITY is
+ * returned (retry with cap = out_offsets[n]). */
+int ita_batched_scan(ita_handle* h, const uint8_t* const* text, const int32_t* len, int32_t n,
+                     int32_t* out_offsets, int32_t* out_ids, int64_t cap);
+
+/* Scan n strings already in device memory: d_text holds the concatenated
+ * bytes and d_offsets (int64, n+1) the string boundaries.  kind: ITA_KIND_*;
+ * mode: ITA_MODE_* (engine only).  stream: a hipStream_t, or NULL for the
+ * handle's own stream.  use_graph != 0 captures the launch sequence in a
+ * hipGraph on first use and replays it for the same (buffers, n, kind,
+ * mode).  Results stay on the device until ita_collect.  No reference
+ * counterpart: the reference always copies inputs per call. */
+int ita_scan_device(ita_handle* h, const uint8_t* d_text, const int64_t* d_offsets, int32_t n, int32_t kind,
+                    int32_t mode, void* stream, int32_t use_graph);
+
+/* Copy the last ita_scan_device results to the host and map accepted states
+ * to rule ids (layout as ita_batched_scan). */
+int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids, int64_t cap);
+
+/* Kernel time (ms, hipEvents) of the last scan and steps per method
+ * (iNFA, TKO, AS) of the last engine scan. */
+int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3);
+
+/* Engine launch variant: waves (strings) per workgroup in {1,2,4}, active-list
+ * capacity in {256,512,1024}; plain_bs in {64,128,256} for the plain kernel. */
+int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs);
+int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs);
+
+/* Learned step selector: 5,443 packed float weights (layout in
+ * admatch_amd/csrc/device.hpp), evaluated every `every` steps.  weights ==
+ * NULL removes the selector (the heuristic is used). */
+int ita_set_selector(ita_handle* h, const float* weights, int32_t n, int32_t every);
+
+/* Step logging for selector training (ITA_MODE_LOG): room for cap records
+ * of 20 floats, one step sampled in every `stride`. */
+int ita_set_log(ita_handle* h, int64_t cap_records, int32_t stride);
+int64_t ita_fetch_log(ita_handle* h, float* out, int64_t cap_records);
+
+/* Rule-set facts: [0] states, [1] u64 words per vector, [2] transitions,
+ * [3] word-boundary transitions, [4] TKO groups, [5] TKO threshold,
+ * [6] AS row entries, [7] engine LDS bytes per workgroup (current variant),
+ * [8] plain-kernel LDS bytes, [9] CUs (after ita_alloc, else 0),
+ * [10] content hash. */
+int ita_info(ita_handle* h, int64_t* out11);
+
+/* Host copies of the tables, for tests: which = 0 iNFA offsets (258 int32),
+ * 1 iNFA list (2*T uint32), 2 TKO k prefix (258 int32), 3 TKO shifts
+ * (int32), 4 TKO lim (u64, K*W), 5 TKO leftover offsets (258 int32),
+ * 6 TKO leftover (2*n uint32), 7 AS offsets (S*256+1 uint32), 8 AS dst
+ * (uint32), 9 init/persistent/accept vectors (3*W u64).  Returns the
+ * element count; copies min(count, cap) elements when out != NULL. */
+int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap);
+
+/* Message of this thread's last failed call. */
+const char* ita_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ADMATCH_H */

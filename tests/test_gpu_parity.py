@@ -1,0 +1,134 @@
+"""GPU parity: every kernel and selector against the CPU oracle (and the
+golden fixtures), through the C ABI.  Bit-exact on sorted rule-id lists."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden_inputs, golden_json, golden_nfa
+
+pytestmark = pytest.mark.gpu
+
+from admatch_amd import engine as E  # noqa: E402
+
+import oracle  # noqa: E402  (checker only)
+
+# (flags, selector mode) for every scan path reachable through the flags
+PATHS = [
+    ("plain-iNFA", E.INFA_KERNEL),
+    ("TKO", E.TKO_KERNEL),
+    ("AS", E.AS_KERNEL),
+    ("AdMatch", E.AD_MATCHER),
+    ("heuristic", E.AD_MATCHER | E.HEURISTIC_SELECTOR),
+    ("learned-untrained", E.AD_MATCHER | E.LEARNED_SELECTOR),
+]
+ENGINE_MODES = [E.MODE_INFA, E.MODE_TKO, E.MODE_AS, E.MODE_ADMATCH, E.MODE_HEURISTIC, E.MODE_LEARNED]
+
+
+def scan_all(flags, path, buf, off, variant=None):
+    s = E.Scratch(flags, path).alloc_scratch()
+    if variant:
+        s.set_variant(*variant)
+    out = E.unpack(*s.batched_scan_packed(buf, off))
+    s.free_scratch()
+    s.close()
+    return out
+
+
+def first_diff(got, exp):
+    for i, (g, e) in enumerate(zip(got, exp)):
+        if g != e:
+            return i, g, e
+    return None
+
+
+@pytest.mark.parametrize("name,flags", PATHS, ids=[p[0] for p in PATHS])
+@pytest.mark.parametrize("group", ["g0", "g6", "g7", "g9"])
+def test_crs_groups(gold_dir, group, name, flags):
+    path = golden_nfa(f"crs_{group}.nfa.gz", gold_dir)
+    buf, off = golden_inputs("http_inputs.npz")
+    exp = golden_json("crs_expected.json")[group]
+    print(f"[case] crs {group} {name} n={off.size - 1}", flush=True)
+    got = scan_all(flags, path, buf, off)
+    assert got == exp, first_diff(got, exp)
+
+
+@pytest.mark.parametrize("name,flags", PATHS, ids=[p[0] for p in PATHS])
+def test_synthetic_small_vs_re(gold_dir, name, flags):
+    path = golden_nfa("synth_small.nfa.gz", gold_dir)
+    buf, off = golden_inputs("synth_small_inputs.npz")
+    exp = golden_json("synth_small_expected.json")["expected"]
+    print(f"[case] synth-small {name}", flush=True)
+    got = scan_all(flags, path, buf, off)
+    assert got == exp, first_diff(got, exp)
+
+
+def test_rule1_known_answers(gold_dir):
+    path = golden_nfa("crs_rule1.nfa", gold_dir)
+    cases = [(b"x application/xml y", [1]), (b"text/xml", [1]), (b"text/xm", []), (b"", []),
+             (b"application/xmlapplication/xml", [1]), (b"TEXT/XML", []), (b"texttext/xml\x00z", [1])]
+    for name, flags in PATHS:
+        s = E.Scratch(flags, path).alloc_scratch()
+        got = s.batched_scan([c for c, _ in cases])
+        assert got == [e for _, e in cases], name
+        assert s.scan(b"a text/xml") == [1]
+        s.close()
+
+
+@pytest.mark.parametrize("cfg,n", [("small", 1024), ("medium", 2048), ("large", 384)])
+def test_synthetic_configs_vs_oracle(workloads, cfg, n):
+    w, path = workloads(cfg, n)
+    buf, off = w.inputs.buf, w.inputs.offsets
+    exp = oracle.OracleNFA(path).batch_scan(buf, off, nthreads=os.cpu_count() or 4)
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+    kinds = [(E.KIND_ENGINE, m) for m in ENGINE_MODES]
+    if s.info()["plain_lds_bytes"] <= 160 * 1024:
+        kinds.append((E.KIND_PLAIN, E.MODE_INFA))
+    for kind, mode in kinds:
+        print(f"[case] {cfg} n={n} kind={kind} mode={E.MODE_NAMES[mode]}", flush=True)
+        s.scan_device(d_buf.ptr, d_off.ptr, off.size - 1, kind, mode)
+        got = E.unpack(*s.collect(off.size - 1))
+        assert got == exp, (kind, mode, first_diff(got, exp))
+    s.close()
+
+
+def test_edge_strings(workloads):
+    w, path = workloads("small", 64)
+    rng = np.random.default_rng(7)
+    strs = [b"", b"a", b"\x00", b"\x00" * 300, bytes(rng.integers(0, 256, 1000, dtype=np.uint8)),
+            b"z" * 5000, w.inputs.string(0) * 9]
+    for i in range(40):  # lengths around the 256-byte window and 64-lane boundaries
+        strs.append(bytes(rng.integers(32, 127, 250 + i * 3, dtype=np.uint8)))
+    strs += [w.inputs.string(i) for i in range(64)]
+    o = oracle.OracleNFA(path)
+    exp = [o.scan(s) for s in strs]
+    for name, flags in PATHS:
+        got = scan_all(flags, path, *E.pack_strings(strs))
+        assert got == exp, (name, first_diff(got, exp))
+
+
+@pytest.mark.parametrize("wpb", [1, 2, 4])
+@pytest.mark.parametrize("cap", [256, 512, 1024])
+def test_variants_agree(workloads, wpb, cap):
+    w, path = workloads("medium", 512)
+    buf, off = w.inputs.buf, w.inputs.offsets
+    exp = oracle.OracleNFA(path).batch_scan(buf, off, nthreads=os.cpu_count() or 4)
+    for flags in (E.AD_MATCHER, E.AS_KERNEL, E.TKO_KERNEL):
+        got = scan_all(flags, path, buf, off, variant=(wpb, cap, 64))
+        assert got == exp, (wpb, cap, flags, first_diff(got, exp))
+
+
+def test_graph_replay_and_append(workloads):
+    w, path = workloads("small", 512)
+    buf, off = w.inputs.buf, w.inputs.offsets
+    exp = oracle.OracleNFA(path).batch_scan(buf, off, nthreads=4)
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+    for rep in range(4):
+        s.scan_device(d_buf.ptr, d_off.ptr, off.size - 1, E.KIND_ENGINE, E.MODE_HEURISTIC, use_graph=True)
+        got = E.unpack(*s.collect(off.size - 1))
+        assert got == exp, rep
+        # interleave eager work between replays
+        assert s.batched_scan([w.inputs.string(1)]) == [exp[1]]
+    s.close()
