@@ -7,6 +7,7 @@
 
 #include "../../include/admatch.h"
 #include "engine.hpp"
+#include "mlp.hpp"
 
 using admatch::Engine;
 
@@ -264,6 +265,25 @@ int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap) {
         }
         default: return fail(ITA_ERR_ARG, "bad table id");
     }
+}
+
+int ita_selector_train(const float* features, const float* targets, const float* mask, int64_t n, int32_t epochs,
+                       int32_t batch, float lr, uint64_t seed, float* weights_inout, float* loss_history) {
+    std::string err;
+    return status(admatch::mlp_train(features, targets, mask, n, epochs, batch, lr, seed, weights_inout, loss_history,
+                                     err),
+                  err);
+}
+
+int ita_selector_grad(const float* features, const float* targets, const float* mask, int64_t n,
+                      const float* weights, float* grad_out, float* loss_out) {
+    std::string err;
+    return status(admatch::mlp_grad(features, targets, mask, n, weights, grad_out, loss_out, err), err);
+}
+
+int ita_selector_forward(const float* features, int64_t n, const float* weights, float* out) {
+    std::string err;
+    return status(admatch::mlp_forward(features, n, weights, out, err), err);
 }
 
 }  // extern "C"

@@ -40,7 +40,7 @@ struct ScanArgs {
     unsigned long long* pool_top;  // [1] (zeroed before launch)
     int32_t* overflow;        // [1] set when the pool is exhausted
     const float* mlp;         // learned selector weights or null
-    int32_t mlp_every;        // evaluate the selector every k steps
+    int32_t mlp_every;        // >0: evaluate every k steps; 0: per-byte decision cache
     // step logging (MODE_LOG): per sampled step 16 features + 3 times
     float* log_buf;
     int64_t log_cap;

@@ -44,7 +44,7 @@ int Engine::load(const char* path, std::string& err) {
 }
 
 size_t Engine::engine_lds_bytes(const Variant& v) const {
-    return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap)) * 8;
+    return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap) + 32) * 8;
 }
 
 size_t Engine::plain_lds_bytes() const { return size_t(3) * g_.W * 8 + 258 * 4; }
@@ -179,7 +179,7 @@ ScanArgs Engine::make_args(const uint8_t* d_text, const int64_t* d_off, int32_t 
     a.pool_top = reinterpret_cast<unsigned long long*>(d_ctr_ + kCtrPoolTop);
     a.overflow = d_ctr_ + kCtrOverflow;
     a.mlp = d_mlp_;
-    a.mlp_every = mlp_every_ > 0 ? mlp_every_ : 1;
+    a.mlp_every = mlp_every_ >= 0 ? mlp_every_ : 0;
     a.log_buf = d_log_;
     a.log_cap = log_cap_;
     a.log_top = reinterpret_cast<unsigned long long*>(d_ctr_ + kCtrLogTop);
@@ -405,7 +405,7 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
         ADM_HIP(hipMalloc(&d_mlp_, sizeof(float) * kMlpParams));
     }
     ADM_HIP(hipMemcpy(d_mlp_, w, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
-    if (every > 0) mlp_every_ = every;
+    mlp_every_ = every >= 0 ? every : 0;
     drop_graphs();
     return ADM_OK;
 }

@@ -68,7 +68,10 @@ class Engine {
     int scan_host(const uint8_t* const* text, const int32_t* len, int32_t n, KernelKind kind, int32_t mode,
                   std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err);
 
-    void set_variant(const Variant& v) { var_ = v; graphs_.clear(); }
+    void set_variant(const Variant& v) {
+        var_ = v;
+        drop_graphs();
+    }
     const Variant& variant() const { return var_; }
     int set_mlp(const float* w, int32_t n, int32_t every, std::string& err);
     bool has_mlp() const { return d_mlp_ != nullptr; }
@@ -106,7 +109,7 @@ class Engine {
     int32_t* d_ctr_ = nullptr;  // counters (see kCtr*)
     int32_t* h_ctr_ = nullptr;  // pinned mirror
     float* d_mlp_ = nullptr;
-    int32_t mlp_every_ = 8;
+    int32_t mlp_every_ = 0;  // 0: per-wave decision cache
     float* d_log_ = nullptr;
     int64_t log_cap_ = 0;
     int32_t log_stride_ = 64;

@@ -292,7 +292,6 @@ __device__ int as_rows(const DevGraph& g, int c, const uint32_t* cl, int n_cur, 
 struct StepCtx {
     int c, n_act, sparse, T, K, Lft, last_m;
     int64_t pos, len;
-    float last_cycles;
 };
 
 __device__ __forceinline__ int heuristic_choice(const DevGraph& g, const StepCtx& s) {
@@ -317,7 +316,7 @@ __device__ __forceinline__ void make_features(const DevGraph& g, const StepCtx& 
     f[11] = s.last_m == M_INFA ? 1.f : 0.f;
     f[12] = s.last_m == M_TKO ? 1.f : 0.f;
     f[13] = s.last_m == M_AS ? 1.f : 0.f;
-    f[14] = __log2f(1.f + s.last_cycles);
+    f[14] = __log2f(1.f + float(s.len - s.pos));
     f[15] = __log2f(1.f + float(s.K) * float(g.W));
 }
 
@@ -355,7 +354,8 @@ __device__ int mlp_choice(const float* w, const float* f, bool allow_as) {
 // ------------------------------------------------------------ engine
 template <int CAP>
 struct Lane {
-    static constexpr size_t words(int W) { return size_t(2) * W + CAP; }  // u64 words per wave
+    // u64 words per wave: two bitmaps, two lists, 256-byte decision cache
+    static constexpr size_t words(int W) { return size_t(2) * W + CAP + 32; }
 };
 
 // Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
@@ -401,7 +401,7 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
 
 template <int CAP>
 __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
-                            uint32_t* listA, uint32_t* listB, int* msteps) {
+                            uint32_t* listA, uint32_t* listB, uint8_t* dcache, int* msteps) {
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int64_t L = a.offsets[sid + 1] - beg;
@@ -444,7 +444,6 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     StepCtx ctx;
     ctx.len = L;
     ctx.last_m = M_INFA;
-    ctx.last_cycles = 0.f;
     int cached = -1;
     for (int64_t b = 0; b < L; ++b) {
         const int c = int(win.at(b));
@@ -464,7 +463,10 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             case MODE_AS: m = sparse ? M_AS : M_INFA; break;
             case MODE_ADMATCH: m = (sparse && ctx.K > n_act) ? M_AS : M_TKO; break;
             case MODE_LEARNED:
-                if (a.mlp != nullptr) {
+                if (a.mlp == nullptr) {
+                    m = heuristic_choice(g, ctx);
+                } else if (a.mlp_every > 0) {
+                    // re-evaluate every mlp_every steps
                     if (cached < 0 || (b % a.mlp_every) == 0) {
                         float f[kFeatures];
                         make_features(g, ctx, f);
@@ -473,7 +475,19 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                     m = cached;
                     if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
                 } else {
-                    m = heuristic_choice(g, ctx);
+                    // per-wave decision cache keyed by the byte, tagged with
+                    // the active-count bucket and list validity
+                    const int tag = ((31 - __clz(n_act + 1)) << 1) | int(sparse);
+                    const int e = dcache[c];
+                    if ((e >> 2) == tag) {
+                        m = e & 3;
+                    } else {
+                        float f[kFeatures];
+                        make_features(g, ctx, f);
+                        m = mlp_choice(a.mlp, f, sparse);
+                        if (lane_id() == 0) dcache[c] = uint8_t((tag << 2) | m);
+                        wave_sync();
+                    }
                 }
                 break;
             case MODE_LOG:
@@ -499,6 +513,11 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
                 n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp);
+                wave_sync();
+                if (!fsp) {  // a bitmap-only result pays for its list rebuild
+                    n_fut = uniform(rebuild_list<CAP>(fut, W, fl));
+                    fsp = n_fut <= CAP;
+                }
                 wave_sync();
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t1 = clock64();
@@ -565,7 +584,9 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
     uint64_t* bitsB = base + g.W;
     uint32_t* listA = reinterpret_cast<uint32_t*>(base + 2 * g.W);
     uint32_t* listB = listA + CAP;
+    uint8_t* dcache = reinterpret_cast<uint8_t*>(base + 2 * g.W + CAP);
     for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
+    for (int j = lane_id(); j < 32; j += 64) base[2 * g.W + CAP + j] = ~0ull;  // no cached decisions
     wave_sync();
     int msteps[3] = {0, 0, 0};  // steps per method, this wave
     for (;;) {
@@ -574,7 +595,7 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
         if (idx >= a.n) break;
         const int sid = a.order ? a.order[idx] : idx;
-        scan_string<CAP>(g, a, sid, bitsA, bitsB, listA, listB, msteps);
+        scan_string<CAP>(g, a, sid, bitsA, bitsB, listA, listB, dcache, msteps);
     }
     if (a.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)

@@ -41,7 +41,8 @@ LOG_RECORD = 20
 EXPORTED = [
     "ita_create", "ita_alloc", "ita_free", "ita_destroy", "ita_scan", "ita_batched_scan", "ita_scan_device",
     "ita_collect", "ita_last_stats", "ita_set_variant", "ita_get_variant", "ita_set_selector", "ita_set_log",
-    "ita_fetch_log", "ita_info", "ita_table", "ita_last_error",
+    "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
+    "ita_selector_forward",
 ]
 
 _LIB = None
@@ -197,7 +198,7 @@ class Scratch:
         _err(lib().ita_get_variant(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
         return a.value, b.value, c.value
 
-    def set_selector(self, weights: Optional[np.ndarray], every: int = 8):
+    def set_selector(self, weights: Optional[np.ndarray], every: int = 0):
         if weights is None:
             _err(lib().ita_set_selector(self._h, None, 0, every))
             return

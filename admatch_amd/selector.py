@@ -1,0 +1,171 @@
+"""Learned per-step kernel selector: logging, GPU training, packaging.
+
+The engine's MODE_LOG runs all three step methods (iNFA, TKO, AS) on the
+same state for one sampled step in ``stride`` and records the step's 16
+features and each method's cycle count.  A 16 -> 64 -> 64 -> 3 MLP is then
+trained on the GPU (ita_selector_train: the project's own forward/backward
+and Adam kernels) to predict log2 cycles per method; at scan time the
+engine evaluates it inside the step loop and runs the method with the
+lowest prediction (the heuristic decides when the network picks the active-
+state method for a state held only as a bitmap).  Results do not depend on
+the choice: every method computes the same next state set.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import engine as E
+
+F, H, O = 16, 64, 3
+N_PARAMS = E.MLP_PARAMS
+WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector_weights.npy")
+
+FEATURE_NAMES = [
+    "log2(1+active)", "active %", "log2(1+|T_c|)", "k_c", "log2(1+leftover_c)", "log2(W)", "word(c)",
+    "c/32", "pos/len", "log2(1+len)", "list valid", "last=iNFA", "last=TKO", "last=AS", "log2(1+last cycles)",
+    "log2(1+k_c*W)",
+]
+
+
+def _sig():
+    L = E.lib()
+    vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+    L.ita_selector_train.restype = i32
+    L.ita_selector_train.argtypes = [vp, vp, vp, i64, i32, i32, ctypes.c_float, ctypes.c_uint64, vp, vp]
+    L.ita_selector_grad.restype = i32
+    L.ita_selector_grad.argtypes = [vp, vp, vp, i64, vp, vp, vp]
+    L.ita_selector_forward.restype = i32
+    L.ita_selector_forward.argtypes = [vp, i64, vp, vp]
+    return L
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+@dataclass
+class Dataset:
+    x: np.ndarray     # [n,16] float32 features
+    y: np.ndarray     # [n,3] log2 cycles (0 where not timed)
+    mask: np.ndarray  # [n,3] 1 where timed
+
+    @property
+    def n(self):
+        return self.x.shape[0]
+
+    def best(self) -> np.ndarray:
+        t = np.where(self.mask > 0, self.y, np.inf)
+        return t.argmin(1)
+
+
+def from_log(records: np.ndarray) -> Dataset:
+    """Log records (20 floats: 16 features, 3 cycle counts or -1, chosen)."""
+    x = records[:, :F].astype(np.float32)
+    t = records[:, F:F + O].astype(np.float64)
+    mask = (t > 0).astype(np.float32)
+    y = np.where(t > 0, np.log2(np.maximum(t, 1.0)), 0.0).astype(np.float32)
+    return Dataset(x=x, y=y, mask=mask)
+
+
+def collect(scratch: E.Scratch, d_buf, d_off, n: int, cap: int = 1 << 20, stride: int = 16) -> np.ndarray:
+    scratch.set_log(cap, stride)
+    scratch.scan_device(d_buf.ptr, d_off.ptr, n, E.KIND_ENGINE, E.MODE_LOG)
+    scratch.collect(n)
+    rec = scratch.fetch_log(cap)
+    scratch.set_log(0, stride)
+    return rec
+
+
+def init_weights(seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    w = np.zeros(N_PARAMS, dtype=np.float32)
+    o = 0
+    for fan_in, fan_out in ((F, H), (H, H)):
+        w[o:o + fan_in * fan_out] = rng.normal(0, np.sqrt(2.0 / fan_in), fan_in * fan_out)
+        o += fan_in * fan_out + fan_out
+    w[o:o + O * H] = rng.normal(0, np.sqrt(1.0 / H), O * H)
+    return w
+
+
+def fold_normalisation(w: np.ndarray, mu: np.ndarray, sd: np.ndarray) -> np.ndarray:
+    """Weights trained on (x-mu)/sd -> weights that take raw x."""
+    w = w.astype(np.float64).copy()
+    W1 = w[:F * H].reshape(F, H)
+    b1 = w[F * H:F * H + H]
+    b1 -= (mu / sd) @ W1
+    W1 /= sd[:, None]
+    w[:F * H] = W1.ravel()
+    w[F * H:F * H + H] = b1
+    return w.astype(np.float32)
+
+
+def train(ds: Dataset, epochs: int = 40, batch: int = 512, lr: float = 2e-3, seed: int = 0):
+    """Trains on the GPU; returns (packed weights for raw features, loss history)."""
+    mu = ds.x.mean(0).astype(np.float64)
+    sd = ds.x.std(0).astype(np.float64)
+    sd[sd < 1e-6] = 1.0
+    xn = ((ds.x - mu) / sd).astype(np.float32)
+    # targets centred per method for a better-conditioned start
+    w = init_weights(seed)
+    w[-O:] = np.array([ds.y[ds.mask[:, m] > 0, m].mean() if (ds.mask[:, m] > 0).any() else 0.0
+                       for m in range(O)], dtype=np.float32)
+    hist = np.zeros(epochs, dtype=np.float32)
+    rc = _sig().ita_selector_train(_p(np.ascontiguousarray(xn)), _p(np.ascontiguousarray(ds.y)),
+                                   _p(np.ascontiguousarray(ds.mask)), ds.n, epochs, batch, lr, seed, _p(w), _p(hist))
+    E._err(rc)
+    return fold_normalisation(w, mu, sd), hist
+
+
+def gpu_grad(x, y, mask, w):
+    g = np.zeros(N_PARAMS, dtype=np.float32)
+    loss = np.zeros(1, dtype=np.float32)
+    x, y, mask, w = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, mask, w))
+    E._err(_sig().ita_selector_grad(_p(x), _p(y), _p(mask), x.shape[0], _p(w), _p(g), _p(loss)))
+    return float(loss[0]), g
+
+
+def gpu_forward(x, w):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    w = np.ascontiguousarray(w, dtype=np.float32)
+    out = np.zeros((x.shape[0], O), dtype=np.float32)
+    E._err(_sig().ita_selector_forward(_p(x), x.shape[0], _p(w), _p(out)))
+    return out
+
+
+def choose(pred: np.ndarray, allow_as: np.ndarray) -> np.ndarray:
+    p = pred.copy()
+    p[~allow_as, 2] = np.inf
+    return p.argmin(1)
+
+
+def regret(ds: Dataset, choice: np.ndarray) -> dict:
+    """Cycles of the chosen methods vs the per-step best, over timed steps."""
+    t = np.where(ds.mask > 0, np.exp2(ds.y.astype(np.float64)), np.inf)
+    picked = t[np.arange(ds.n), choice]
+    best = t.min(1)
+    ok = np.isfinite(picked)
+    return {"picked_cycles": float(picked[ok].sum()), "best_cycles": float(best[ok].sum()),
+            "accuracy": float((choice == t.argmin(1)).mean())}
+
+
+def heuristic_choice(ds: Dataset, W: int) -> np.ndarray:
+    """The engine's fallback rule on logged features (AdMatch_kernel.cu:87
+    plus an iNFA cost term), for offline comparison."""
+    active = np.rint(np.exp2(ds.x[:, 0]) - 1)
+    T = np.rint(np.exp2(ds.x[:, 2]) - 1)
+    K = ds.x[:, 3]
+    Lft = np.rint(np.exp2(ds.x[:, 4]) - 1)
+    sparse = ds.x[:, 10] > 0
+    out = np.where(T <= K * W + Lft, 0, 1)
+    out[sparse & (K > active)] = 2
+    return out
+
+
+def load_weights(path: Optional[str] = None) -> Optional[np.ndarray]:
+    path = path or WEIGHTS_PATH
+    return np.load(path) if os.path.exists(path) else None

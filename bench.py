@@ -29,7 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SELECTOR_PATH = os.path.join(ROOT, "admatch_amd", "selector", "selector_weights.npy")
+SELECTOR_PATH = os.path.join(ROOT, "admatch_amd", "data", "selector_weights.npy")
 
 
 def log(msg: str) -> None:
@@ -108,7 +108,7 @@ def main():
     s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
     info = s.info()
     tuned = None
-    tune_path = os.path.join(ROOT, "admatch_amd", "selector", "autotune.json")
+    tune_path = os.path.join(ROOT, "admatch_amd", "data", "autotune.json")
     if os.path.exists(tune_path):
         with open(tune_path) as f:
             tuned = json.load(f).get(str(info["hash"]))
@@ -116,7 +116,7 @@ def main():
         s.set_variant(tuned["wpb"], tuned["cap"], tuned.get("plain_bs", 64))
     learned_ok = False
     if os.path.exists(SELECTOR_PATH):
-        s.set_selector(np.load(SELECTOR_PATH), every=8)
+        s.set_selector(np.load(SELECTOR_PATH), every=0)
         learned_ok = True
     d_buf, d_off = E.DeviceArray(w.inputs.buf), E.DeviceArray(w.inputs.offsets)
 

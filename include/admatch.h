@@ -112,8 +112,11 @@ int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs);
 int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs);
 
 /* Learned step selector: 5,443 packed float weights (layout in
- * admatch_amd/csrc/device.hpp), evaluated every `every` steps.  weights ==
- * NULL removes the selector (the heuristic is used). */
+ * admatch_amd/csrc/device.hpp).  every == 0 (default): each wave caches the
+ * network's choice per input byte, tagged with the active-count bucket and
+ * list validity, and evaluates the network on a miss; every > 0: evaluate
+ * every `every` steps.  weights == NULL removes the selector (the heuristic
+ * is used). */
 int ita_set_selector(ita_handle* h, const float* weights, int32_t n, int32_t every);
 
 /* Step logging for selector training (ITA_MODE_LOG): room for cap records
@@ -135,6 +138,19 @@ int ita_info(ita_handle* h, int64_t* out11);
  * (uint32), 9 init/persistent/accept vectors (3*W u64).  Returns the
  * element count; copies min(count, cap) elements when out != NULL. */
 int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap);
+
+/* Learned-selector training on the GPU (no counterpart in the reference):
+ * samples are 16 features, 3 per-method targets (log2 step cost) and 3
+ * mask values (1 = the method was timed).  Weights use the packed layout
+ * of ita_set_selector.  ita_selector_train runs Adam on shuffled minibatches
+ * (weights_inout: initial in, trained out; loss_history[epochs], optional);
+ * ita_selector_grad returns the loss gradient at the given weights;
+ * ita_selector_forward evaluates the network (out[n][3]). */
+int ita_selector_train(const float* features, const float* targets, const float* mask, int64_t n, int32_t epochs,
+                       int32_t batch, float lr, uint64_t seed, float* weights_inout, float* loss_history);
+int ita_selector_grad(const float* features, const float* targets, const float* mask, int64_t n,
+                      const float* weights, float* grad_out, float* loss_out);
+int ita_selector_forward(const float* features, int64_t n, const float* weights, float* out);
 
 /* Message of this thread's last failed call. */
 const char* ita_last_error(void);

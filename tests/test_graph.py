@@ -1,0 +1,182 @@
+"""Host graph model of the product (libadmatch.so, no GPU needed) against
+the oracle's parse and the restated TKO partition."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden_nfa
+from nfa_gen import random_nfa, shift_nfa
+
+import oracle  # checker
+from admatch_amd import engine as E
+from admatch_amd import merge
+
+FIXTURES = ["crs_rule1.nfa", "crs_g0.nfa.gz", "crs_g6.nfa.gz", "crs_g7.nfa.gz", "crs_g9.nfa.gz", "synth_small.nfa.gz"]
+REF = "/root/reference"
+
+
+def to64(words32: np.ndarray) -> np.ndarray:
+    w = np.concatenate([words32, np.zeros(len(words32) % 2, np.uint32)])
+    return w.view(np.uint64)
+
+
+def same_as_oracle(path: str):
+    o = oracle.OracleNFA(path)
+    s = E.ItaScratch(E.INFA_KERNEL, path)
+    assert (s.table(0).astype(np.int64) == o.offsets()).all()
+    assert (s.table(1).reshape(-1, 2) == o.transitions()).all()
+    v = s.table(9).reshape(3, -1)
+    for a, b in zip(o.vectors(), v):
+        assert (to64(a) == b).all()
+    info = s.info()
+    assert info["states"] == o.S and info["wb_transitions"] == o.n_wb
+    return s, o
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_parse_matches_oracle(gold_dir, name):
+    same_as_oracle(golden_nfa(name, gold_dir))
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_parse_random(tmp_path, seed):
+    p = tmp_path / "r.nfa"
+    p.write_text(random_nfa(seed, S=70, n_edges=300))
+    same_as_oracle(str(p))
+
+
+def check_tko(s, o):
+    k_per, groups, leftover = oracle.tko_partition(o)
+    koff = s.table(2)
+    assert (np.diff(koff) == k_per).all()
+    shifts = s.table(3)
+    W = s.info()["words"]
+    lim = s.table(4).reshape(-1, W) if shifts.size else np.zeros((0, W), np.uint64)
+    loff = s.table(5)
+    left = s.table(6).reshape(-1, 2)
+    assert s.info()["tko_threshold"] == o.S // 16
+    for c in range(257):
+        gs = groups.get(c, [])
+        for j, (d, srcs) in enumerate(gs):
+            i = koff[c] + j
+            assert shifts[i] == d, (c, j)
+            bits = np.unpackbits(lim[i].view(np.uint8), bitorder="little")
+            assert (np.nonzero(bits)[0] == srcs).all(), (c, d)
+        lo = left[loff[c]:loff[c + 1]].astype(np.int64)
+        exp = leftover[c]
+        assert lo.shape[0] == exp.shape[0] and (lo == exp).all(), c
+
+
+@pytest.mark.parametrize("name", ["crs_g6.nfa.gz", "crs_g7.nfa.gz", "synth_small.nfa.gz"])
+def test_tko_partition_fixtures(gold_dir, name):
+    s, o = same_as_oracle(golden_nfa(name, gold_dir))
+    check_tko(s, o)
+
+
+def test_tko_partition_negative_offsets(gold_dir):
+    # group 7 of script/run.sh has top-k groups with negative offsets
+    s, o = same_as_oracle(golden_nfa("crs_g7.nfa.gz", gold_dir))
+    shifts = s.table(3)
+    assert (shifts < 0).sum() > 1000 and ((shifts % 32) != 0).sum() > 1000
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_tko_partition_shift_nfa(tmp_path, seed):
+    p = tmp_path / "s.nfa"
+    p.write_text(shift_nfa(seed))
+    s, o = same_as_oracle(str(p))
+    check_tko(s, o)
+    # every stressed offset became a group on some symbol
+    from nfa_gen import SHIFT_OFFSETS
+    assert set(SHIFT_OFFSETS) <= set(s.table(3).tolist())
+
+
+def test_as_rows_equal_lists(gold_dir):
+    s, o = same_as_oracle(golden_nfa("crs_g9.nfa.gz", gold_dir))
+    S = o.S
+    aoff = s.table(7).astype(np.int64)
+    adst = s.table(8)
+    off = o.offsets()
+    tr = o.transitions()
+    assert aoff.size == S * 256 + 1 and aoff[-1] == off[256]
+    for c in (0, 47, 97, 101, 255):
+        t = tr[off[c]:off[c + 1]]
+        rows = [(sv, adst[aoff[sv * 256 + c]: aoff[sv * 256 + c + 1]]) for sv in np.unique(t[:, 0])]
+        got = np.array([(sv, d) for sv, ds in rows for d in ds], dtype=np.uint32).reshape(-1, 2)
+        assert (got == t).all()
+
+
+@pytest.mark.parametrize("text,err", [
+    ("", "no state count"),
+    ("abc\n", "bad state count"),
+    ("3\n0 -> 5 : 97\n", "out of range"),
+    ("3\n0 -> 1 : 300\n", "symbol out of range"),
+    ("3\n0 -> 1 : x\n", "bad symbol"),
+    ("3\n-1 -> 1 : 97\n", "bad state id"),
+])
+def test_malformed_files(tmp_path, text, err):
+    p = tmp_path / "bad.nfa"
+    p.write_text(text)
+    with pytest.raises(E.AdmatchError) as ei:
+        E.ItaScratch(E.INFA_KERNEL, str(p))
+    assert ei.value.rc == -2 and err in str(ei.value)
+    with pytest.raises(ValueError):
+        oracle.OracleNFA(str(p))
+
+
+def test_tolerated_lines(tmp_path):
+    # unparsable lines are skipped (transition_graph.cpp:146,292), CRLF is
+    # stripped, accept lines without rules and ids beyond the count are ignored
+    text = "# c\r\n4\r\n0 : initial\r\n0 -> 1 : 97\r\nnonsense\r\n1 : accepting\r\n1 : accepting 3 4\r\n" \
+           "1 : accepting 9\r\n9 : initial\r\n0 -> 0 : 0|255\r\n"
+    p = tmp_path / "ok.nfa"
+    p.write_text(text)
+    s, o = same_as_oracle(str(p))
+    assert o.n_bad_lines == 1 and o.n_accept_map == 1
+    off, rules = o.accept_map()
+    assert rules[off[1]:off[2]].tolist() == [3, 4]  # first line with rules wins
+    assert o.scan(b"xa") == [3, 4]
+
+
+def test_missing_file():
+    with pytest.raises(E.AdmatchError) as ei:
+        E.ItaScratch(E.INFA_KERNEL, "/nonexistent/file.nfa")
+    assert ei.value.rc == -2
+
+
+@pytest.mark.parametrize("flags", [0, E.INFA_KERNEL | E.TKO_KERNEL, E.AD_MATCHER | E.AS_KERNEL,
+                                   E.LEARNED_SELECTOR | E.INFA_KERNEL,
+                                   E.AD_MATCHER | E.LEARNED_SELECTOR | E.HEURISTIC_SELECTOR])
+def test_flag_verification(gold_dir, flags):
+    # exactly one kernel bit, or AD_MATCHER alone (Interface.cu:12-18)
+    with pytest.raises(E.AdmatchError) as ei:
+        E.Scratch(flags, golden_nfa("crs_rule1.nfa", gold_dir))
+    assert ei.value.rc == -1
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference checkout not mounted")
+def test_merge_group_sizes():
+    # state totals of script/run.sh groups reconstructed with the merge rules
+    import re
+    text = open(os.path.join(REF, "script", "run.sh")).read()
+    lists = [[int(x) for x in l.split(",")] for l in re.findall(r"for i in \{([0-9,]+)\}", text)]
+    expect = {0: 1000, 1: 1021, 2: 1010, 3: 980, 4: 1993, 5: 2005, 6: 1997, 7: 3367, 9: 4787}
+    for g, states in expect.items():
+        paths = [os.path.join(REF, "data", "crs", f"rule{i}.nfa") for i in lists[g]]
+        merged = merge.merge_files([p for p in paths if os.path.exists(p)])
+        assert int(merged.split("\n")[1]) == states, g
+
+
+def test_merge_rule_ids(tmp_path):
+    a = tmp_path / "rule10.nfa"
+    b = tmp_path / "rule9.nfa"
+    a.write_text("# x\r\n2\r\n0 : initial\r\n0 -> 1 : 97\r\n1 : accepting 55\r\n")
+    b.write_text("3\n0 : initial\n0 -> 2 : 98\n2 : accepting 1\n")
+    text = merge.merge_files([str(a), str(b)])
+    p = tmp_path / "m.nfa"
+    p.write_text(text)
+    o = oracle.OracleNFA(str(p))
+    assert o.S == 5
+    # natural sort puts rule9 first: its rule id is 1, rule10's is 2
+    assert o.scan(b"b") == [1] and o.scan(b"a") == [2] and o.scan(b"xa") == []
