@@ -8,6 +8,7 @@
 #include "../../include/admatch.h"
 #include "engine.hpp"
 #include "mlp.hpp"
+#include "serve.hpp"
 
 using admatch::Engine;
 
@@ -153,16 +154,25 @@ int ita_scan_device(ita_handle* h, const uint8_t* d_text, const int64_t* d_offse
     if (kind != ITA_KIND_PLAIN && kind != ITA_KIND_ENGINE) return fail(ITA_ERR_ARG, "bad kind");
     if (mode < ITA_MODE_INFA || mode > ITA_MODE_LOG) return fail(ITA_ERR_ARG, "bad mode");
     std::string err;
-    return status(h->eng.scan_device(d_text, d_offsets, n, admatch::KernelKind(kind), mode,
-                                     static_cast<hipStream_t>(stream), use_graph != 0, err),
-                  err);
+    admatch::Context& c = h->eng.default_context();
+    if (int rc = c.init(err)) return fail(rc, err);
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    if (user) {  // order after the caller's stream, and make it wait for the scan
+        if (hipEventRecord(c.done, user) != hipSuccess || hipStreamWaitEvent(c.stream, c.done, 0) != hipSuccess)
+            return fail(ITA_ERR_HIP, "cannot order after the caller's stream");
+    }
+    int rc = h->eng.scan_device(d_text, d_offsets, n, admatch::KernelKind(kind), mode, use_graph != 0, err);
+    if (rc) return fail(rc, err);
+    if (user && hipStreamWaitEvent(user, c.ev1, 0) != hipSuccess)
+        return fail(ITA_ERR_HIP, "cannot order the caller's stream after the scan");
+    return ITA_OK;
 }
 
 int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids, int64_t cap) {
     if (int rc = check(h)) return rc;
     if (n < 0 || !out_offsets) return fail(ITA_ERR_ARG, "bad output buffers");
     std::string err;
-    if (int rc = h->eng.collect(n, h->off, h->ids, nullptr, err)) return fail(rc, err);
+    if (int rc = h->eng.collect(n, h->off, h->ids, err)) return fail(rc, err);
     std::memcpy(out_offsets, h->off.data(), sizeof(int32_t) * (size_t(n) + 1));
     const int64_t total = h->off.back();
     if (total > cap) return fail(ITA_ERR_CAPACITY, "out_ids too small");
@@ -284,6 +294,41 @@ int ita_selector_grad(const float* features, const float* targets, const float* 
 int ita_selector_forward(const float* features, int64_t n, const float* weights, float* out) {
     std::string err;
     return status(admatch::mlp_forward(features, n, weights, out, err), err);
+}
+
+int ita_serve_trace(ita_handle* h, int32_t kind, int32_t mode, int32_t lanes, int32_t max_batch,
+                    int64_t max_batch_bytes, int64_t state_budget_bytes, const uint8_t* buf, const int64_t* offsets,
+                    const double* arrival_s, int32_t n, double* latency_s, int32_t* out_offsets, int32_t* out_ids,
+                    int64_t cap, double* stats6) {
+    if (int rc = check(h)) return rc;
+    if (kind != ITA_KIND_PLAIN && kind != ITA_KIND_ENGINE) return fail(ITA_ERR_ARG, "bad kind");
+    if (mode < ITA_MODE_INFA || mode > ITA_MODE_LEARNED) return fail(ITA_ERR_ARG, "bad mode");
+    if (!latency_s || !out_offsets) return fail(ITA_ERR_ARG, "null output");
+    if (!h->eng.on_device()) return fail(ITA_ERR_STATE, "rule set not on the device (allocScratch first)");
+    admatch::ServeConfig cfg;
+    cfg.kind = kind;
+    cfg.mode = mode;
+    cfg.lanes = lanes;
+    cfg.max_batch = max_batch;
+    cfg.max_batch_bytes = max_batch_bytes;
+    cfg.state_budget_bytes = state_budget_bytes;
+    admatch::ServeResult r;
+    std::string err;
+    if (int rc = admatch::serve_trace(h->eng, cfg, buf, offsets, arrival_s, n, r, err)) return fail(rc, err);
+    std::memcpy(latency_s, r.latency_s.data(), sizeof(double) * size_t(n));
+    std::memcpy(out_offsets, r.offsets.data(), sizeof(int32_t) * (size_t(n) + 1));
+    if (stats6) {
+        stats6[0] = r.wall_s;
+        stats6[1] = double(r.batches);
+        stats6[2] = r.mean_batch;
+        stats6[3] = double(r.max_inflight_bytes);
+        stats6[4] = double(r.max_lanes_busy);
+        stats6[5] = n ? double(offsets[n] - offsets[0]) : 0.0;
+    }
+    const int64_t total = r.offsets.back();
+    if (total > cap) return fail(ITA_ERR_CAPACITY, "out_ids too small");
+    if (total) std::memcpy(out_ids, r.ids.data(), sizeof(int32_t) * size_t(total));
+    return ITA_OK;
 }
 
 }  // extern "C"

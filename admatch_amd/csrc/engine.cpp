@@ -17,17 +17,131 @@ namespace admatch {
         }                                                                                   \
     } while (0)
 
+template <class T>
+static void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+template <class T>
+static void hfree(T*& p) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+}
+
+// ------------------------------------------------------------------ Context
+Context::~Context() {
+    release();
+    dfree(d_ctr);
+    hfree(h_ctr);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (done) (void)hipEventDestroy(done);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Context::init(std::string& err) {
+    if (stream) return ADM_OK;
+    // default priority: every stream of the engine has the same priority
+    ADM_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    ADM_HIP(hipEventCreate(&ev0));
+    ADM_HIP(hipEventCreate(&ev1));
+    ADM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    ADM_HIP(hipMalloc(&d_ctr, kCtrWords * sizeof(int32_t)));
+    ADM_HIP(hipHostMalloc(&h_ctr, kCtrWords * sizeof(int32_t)));
+    return ADM_OK;
+}
+
+void Context::drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
+}
+
+void Context::release() {
+    drop_graphs();
+    if (stream) (void)hipStreamSynchronize(stream);
+    dfree(d_res_count);
+    dfree(d_res_start);
+    dfree(d_pool);
+    dfree(d_order);
+    dfree(d_text);
+    dfree(d_off);
+    hfree(h_res_count);
+    hfree(h_res_start);
+    hfree(h_text);
+    hfree(h_off);
+    res_n_cap = 0;
+    h_res_cap = 0;
+    pool_cap = 0;
+    text_cap = 0;
+    off_cap = 0;
+}
+
+size_t Context::device_bytes() const {
+    return size_t(res_n_cap) * (4 + 8 + 4) + size_t(pool_cap) * 4 + size_t(text_cap) + size_t(off_cap) * 8 +
+           kCtrWords * 4;
+}
+
+int Context::ensure_results(int32_t n, std::string& err) {
+    if (n > res_n_cap) {
+        drop_graphs();
+        const int32_t cap = std::max<int32_t>(n, 1024);
+        dfree(d_res_count);
+        dfree(d_res_start);
+        dfree(d_order);
+        hfree(h_res_count);
+        hfree(h_res_start);
+        ADM_HIP(hipMalloc(&d_res_count, sizeof(int32_t) * cap));
+        ADM_HIP(hipMalloc(&d_res_start, sizeof(int64_t) * cap));
+        ADM_HIP(hipMalloc(&d_order, sizeof(int32_t) * cap));
+        ADM_HIP(hipHostMalloc(&h_res_count, sizeof(int32_t) * cap));
+        ADM_HIP(hipHostMalloc(&h_res_start, sizeof(int64_t) * cap));
+        res_n_cap = h_res_cap = cap;
+    }
+    const int64_t want = std::max<int64_t>(int64_t(8) * n, int64_t(1) << 16);
+    if (pool_cap < want) {
+        drop_graphs();
+        dfree(d_pool);
+        ADM_HIP(hipMalloc(&d_pool, sizeof(uint32_t) * want));
+        pool_cap = want;
+    }
+    return ADM_OK;
+}
+
+int Context::grow_pool(std::string& err) {
+    const int64_t want = pool_cap * 4;
+    drop_graphs();
+    dfree(d_pool);
+    ADM_HIP(hipMalloc(&d_pool, sizeof(uint32_t) * want));
+    pool_cap = want;
+    return ADM_OK;
+}
+
+int Context::ensure_staging(int64_t bytes, int32_t n, std::string& err) {
+    if (bytes > text_cap) {
+        const int64_t cap = std::max<int64_t>(bytes + (bytes >> 1), 1 << 20);
+        drop_graphs();
+        hfree(h_text);
+        dfree(d_text);
+        ADM_HIP(hipHostMalloc(&h_text, size_t(cap)));
+        ADM_HIP(hipMalloc(&d_text, size_t(cap)));
+        text_cap = cap;
+    }
+    if (n + 1 > off_cap) {
+        const int32_t cap = std::max<int32_t>(n + 1 + (n >> 1), 4096);
+        drop_graphs();
+        hfree(h_off);
+        dfree(d_off);
+        ADM_HIP(hipHostMalloc(&h_off, sizeof(int64_t) * cap));
+        ADM_HIP(hipMalloc(&d_off, sizeof(int64_t) * cap));
+        off_cap = cap;
+    }
+    return ADM_OK;
+}
+
+// ------------------------------------------------------------------ Engine
 Engine::Engine() {}
 
-Engine::~Engine() {
-    release();
-    if (h_ctr_) (void)hipHostFree(h_ctr_);
-    if (h_text_) (void)hipHostFree(h_text_);
-    if (h_off_) (void)hipHostFree(h_off_);
-    if (ev0_) (void)hipEventDestroy(ev0_);
-    if (ev1_) (void)hipEventDestroy(ev1_);
-    if (stream_) (void)hipStreamDestroy(stream_);
-}
+Engine::~Engine() { release(); }
 
 int Engine::load(const char* path, std::string& err) {
     if (!path) {
@@ -49,35 +163,39 @@ size_t Engine::engine_lds_bytes(const Variant& v) const {
 
 size_t Engine::plain_lds_bytes() const { return size_t(3) * g_.W * 8 + 258 * 4; }
 
-void Engine::drop_graphs() {
-    for (auto& kv : graphs_) (void)hipGraphExecDestroy(kv.second);
-    graphs_.clear();
+void Engine::drop_all_graphs() {
+    ctx_.drop_graphs();
+    for (Context* c : contexts_) c->drop_graphs();
+}
+
+void Engine::set_variant(const Variant& v) {
+    var_ = v;
+    drop_all_graphs();
 }
 
 void Engine::release() {
-    drop_graphs();
-    if (stream_) (void)hipStreamSynchronize(stream_);
+    drop_all_graphs();
+    ctx_.release();
+    for (Context* c : contexts_) c->release();
     for (void* p : dev_allocs_) (void)hipFree(p);
     dev_allocs_.clear();
-    auto f = [](auto*& p) {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-    };
-    f(d_res_count_);
-    f(d_res_start_);
-    f(d_pool_);
-    f(d_order_);
-    f(d_ctr_);
-    f(d_mlp_);
-    f(d_log_);
-    f(d_text_);
-    f(d_off_);
-    res_n_cap_ = 0;
-    pool_cap_ = 0;
-    d_text_cap_ = 0;
-    off_cap_ = 0;
+    table_bytes_ = 0;
+    dfree(d_mlp_);
+    dfree(d_log_);
     log_cap_ = 0;
     dev_ready_ = false;
+}
+
+int Engine::new_context(std::unique_ptr<Context>& out, std::string& err) {
+    auto c = std::make_unique<Context>();
+    if (int rc = c->init(err)) return rc;
+    contexts_.push_back(c.get());
+    out = std::move(c);
+    return ADM_OK;
+}
+
+void Engine::forget_context(Context* c) {
+    contexts_.erase(std::remove(contexts_.begin(), contexts_.end(), c), contexts_.end());
 }
 
 int Engine::upload(std::string& err) {
@@ -86,22 +204,19 @@ int Engine::upload(std::string& err) {
         return ADM_ERR_STATE;
     }
     if (dev_ready_) return ADM_OK;
-    if (!stream_) {
-        ADM_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-        ADM_HIP(hipEventCreate(&ev0_));
-        ADM_HIP(hipEventCreate(&ev1_));
-        int dev = 0;
-        ADM_HIP(hipGetDevice(&dev));
-        ADM_HIP(hipDeviceGetAttribute(&cu_count_, hipDeviceAttributeMultiprocessorCount, dev));
-        int lds = 0;
-        ADM_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
-        max_lds_ = size_t(lds);
-    }
+    if (int rc = ctx_.init(err)) return rc;
+    int dev = 0;
+    ADM_HIP(hipGetDevice(&dev));
+    ADM_HIP(hipDeviceGetAttribute(&cu_count_, hipDeviceAttributeMultiprocessorCount, dev));
+    int lds = 0;
+    ADM_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    max_lds_ = size_t(lds);
     auto put = [&](const void* src, size_t bytes, const void** dst) -> int {
         void* p = nullptr;
         const size_t nb = bytes ? bytes : 8;
         ADM_HIP(hipMalloc(&p, nb));
         dev_allocs_.push_back(p);
+        table_bytes_ += nb;
         if (bytes) ADM_HIP(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
         *dst = p;
         return ADM_OK;
@@ -132,66 +247,39 @@ int Engine::upload(std::string& err) {
 #undef ADM_PUT
     dev_.wb_begin = g_.sym_off[kWordBoundary];
     dev_.wb_count = g_.wb_count();
-    ADM_HIP(hipMalloc(&d_ctr_, kCtrWords * sizeof(int32_t)));
-    if (!h_ctr_) ADM_HIP(hipHostMalloc(&h_ctr_, kCtrWords * sizeof(int32_t)));
     dev_ready_ = true;
     return ADM_OK;
 }
 
-int Engine::ensure_results(int32_t n, std::string& err) {
-    if (n > res_n_cap_) {
-        drop_graphs();
-        const int32_t cap = std::max<int32_t>(n, 1024);
-        if (d_res_count_) (void)hipFree(d_res_count_);
-        if (d_res_start_) (void)hipFree(d_res_start_);
-        if (d_order_) (void)hipFree(d_order_);
-        d_res_count_ = nullptr;
-        d_res_start_ = nullptr;
-        d_order_ = nullptr;
-        ADM_HIP(hipMalloc(&d_res_count_, sizeof(int32_t) * cap));
-        ADM_HIP(hipMalloc(&d_res_start_, sizeof(int64_t) * cap));
-        ADM_HIP(hipMalloc(&d_order_, sizeof(int32_t) * cap));
-        res_n_cap_ = cap;
-    }
-    const int64_t want = std::max<int64_t>(int64_t(8) * n, int64_t(1) << 16);
-    if (pool_cap_ < want) {
-        drop_graphs();
-        if (d_pool_) (void)hipFree(d_pool_);
-        d_pool_ = nullptr;
-        ADM_HIP(hipMalloc(&d_pool_, sizeof(uint32_t) * want));
-        pool_cap_ = want;
-    }
-    return ADM_OK;
-}
-
-ScanArgs Engine::make_args(const uint8_t* d_text, const int64_t* d_off, int32_t n, int32_t mode) {
+ScanArgs Engine::make_args(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, int32_t mode) {
     ScanArgs a{};
     a.text = d_text;
     a.offsets = d_off;
-    a.order = d_order_;
+    a.order = c.d_order;
     a.n = n;
     a.mode = mode;
-    a.queue = d_ctr_ + kCtrQueue;
-    a.res_count = d_res_count_;
-    a.res_start = d_res_start_;
-    a.res_pool = d_pool_;
-    a.pool_cap = pool_cap_;
-    a.pool_top = reinterpret_cast<unsigned long long*>(d_ctr_ + kCtrPoolTop);
-    a.overflow = d_ctr_ + kCtrOverflow;
+    a.queue = c.d_ctr + kCtrQueue;
+    a.res_count = c.d_res_count;
+    a.res_start = c.d_res_start;
+    a.res_pool = c.d_pool;
+    a.pool_cap = c.pool_cap;
+    a.pool_top = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrPoolTop);
+    a.overflow = c.d_ctr + kCtrOverflow;
     a.mlp = d_mlp_;
     a.mlp_every = mlp_every_ >= 0 ? mlp_every_ : 0;
     a.log_buf = d_log_;
     a.log_cap = log_cap_;
-    a.log_top = reinterpret_cast<unsigned long long*>(d_ctr_ + kCtrLogTop);
+    a.log_top = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrLogTop);
     a.log_stride = log_stride_ > 0 ? log_stride_ : 1;
-    a.method_steps = reinterpret_cast<unsigned long long*>(d_ctr_ + kCtrMethods);
+    a.method_steps = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrMethods);
     return a;
 }
 
-int Engine::launch(const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
-                   hipStream_t st, std::string& err) {
-    ScanArgs a = make_args(d_text, d_off, n, mode);
-    ADM_HIP(launch_fill(d_ctr_, 0, kCtrWords, st));
+int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind,
+                   int32_t mode, std::string& err) {
+    ScanArgs a = make_args(c, d_text, d_off, n, mode);
+    hipStream_t st = c.stream;
+    ADM_HIP(launch_fill(c.d_ctr, 0, kCtrWords, st));
     if (n == 0) return ADM_OK;
     if (kind == KIND_PLAIN) {
         const size_t lds = plain_lds_bytes();
@@ -212,7 +300,7 @@ int Engine::launch(const uint8_t* d_text, const int64_t* d_off, int32_t n, Kerne
         err = "logging mode needs set_log() first";
         return ADM_ERR_STATE;
     }
-    ADM_HIP(launch_order(d_off, n, d_ctr_ + kCtrBins, d_ctr_ + kCtrCursors, d_order_, st));
+    ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
     int blocks = 0;
     ADM_HIP(engine_occupancy(var_.wpb, var_.cap, lds, &blocks));
     if (blocks <= 0) {
@@ -226,8 +314,8 @@ int Engine::launch(const uint8_t* d_text, const int64_t* d_off, int32_t n, Kerne
     return ADM_OK;
 }
 
-int Engine::scan_device(const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
-                        hipStream_t st, bool use_graph, std::string& err) {
+int Engine::scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind,
+                        int32_t mode, bool use_graph, std::string& err) {
     if (!dev_ready_) {
         err = "rule set not on the device (allocScratch first)";
         return ADM_ERR_STATE;
@@ -236,18 +324,19 @@ int Engine::scan_device(const uint8_t* d_text, const int64_t* d_off, int32_t n, 
         err = "bad arguments";
         return ADM_ERR_ARG;
     }
-    if (!st) st = stream_;
-    if (int rc = ensure_results(n, err)) return rc;
-    ADM_HIP(hipEventRecord(ev0_, st));
+    if (int rc = c.init(err)) return rc;
+    if (int rc = c.ensure_results(n, err)) return rc;
+    hipStream_t st = c.stream;
+    ADM_HIP(hipEventRecord(c.ev0, st));
     if (use_graph && n > 0) {
         auto key = std::make_tuple(static_cast<const void*>(d_text), static_cast<const void*>(d_off), n,
-                                   int32_t(kind), mode, static_cast<void*>(st));
-        auto it = graphs_.find(key);
-        if (it == graphs_.end()) {
+                                   int32_t(kind), mode);
+        auto it = c.graphs.find(key);
+        if (it == c.graphs.end()) {
             hipGraph_t gr = nullptr;
             ADM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             std::string lerr;
-            int rc = launch(d_text, d_off, n, kind, mode, st, lerr);
+            int rc = launch(c, d_text, d_off, n, kind, mode, lerr);
             hipError_t ec = hipStreamEndCapture(st, &gr);
             if (rc) {
                 if (gr) (void)hipGraphDestroy(gr);
@@ -265,50 +354,86 @@ int Engine::scan_device(const uint8_t* d_text, const int64_t* d_off, int32_t n, 
                 err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
                 return ADM_ERR_HIP;
             }
-            it = graphs_.emplace(key, ex).first;
+            it = c.graphs.emplace(key, ex).first;
         }
         ADM_HIP(hipGraphLaunch(it->second, st));
     } else {
-        if (int rc = launch(d_text, d_off, n, kind, mode, st, err)) return rc;
+        if (int rc = launch(c, d_text, d_off, n, kind, mode, err)) return rc;
     }
-    ADM_HIP(hipEventRecord(ev1_, st));
+    ADM_HIP(hipEventRecord(c.ev1, st));
     return ADM_OK;
 }
 
-int Engine::collect(int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, hipStream_t st,
-                    std::string& err) {
-    if (!st) st = stream_;
-    out_off.assign(size_t(n) + 1, 0);
-    out_ids.clear();
-    ADM_HIP(hipMemcpyAsync(h_ctr_, d_ctr_, kCtrWords * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    std::vector<int32_t> cnt(static_cast<size_t>(n));
-    std::vector<int64_t> start(static_cast<size_t>(n));
-    if (n > 0) {
-        ADM_HIP(hipMemcpyAsync(cnt.data(), d_res_count_, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st));
-        ADM_HIP(hipMemcpyAsync(start.data(), d_res_start_, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st));
+int Engine::stage(Context& c, const uint8_t* const* text, const int32_t* len, int32_t n, std::string& err) {
+    if (n < 0 || (n > 0 && (!text || !len))) {
+        err = "bad arguments";
+        return ADM_ERR_ARG;
     }
-    ADM_HIP(hipStreamSynchronize(st));
+    int64_t total = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        if (len[i] < 0 || (len[i] > 0 && !text[i])) {
+            err = "bad string";
+            return ADM_ERR_ARG;
+        }
+        total += len[i];
+    }
+    if (int rc = c.init(err)) return rc;
+    // the previous use of the staging buffers must have finished
+    ADM_HIP(hipStreamSynchronize(c.stream));
+    if (int rc = c.ensure_staging(total, n, err)) return rc;
+    int64_t o = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        c.h_off[i] = o;
+        if (len[i] > 0) std::memcpy(c.h_text + o, text[i], size_t(len[i]));
+        o += len[i];
+    }
+    c.h_off[n] = o;
+    if (total > 0) ADM_HIP(hipMemcpyAsync(c.d_text, c.h_text, size_t(total), hipMemcpyHostToDevice, c.stream));
+    ADM_HIP(hipMemcpyAsync(c.d_off, c.h_off, sizeof(int64_t) * (size_t(n) + 1), hipMemcpyHostToDevice, c.stream));
+    return ADM_OK;
+}
+
+int Engine::begin_collect(Context& c, int32_t n, std::string& err) {
+    if (n > c.res_n_cap) {
+        err = "collect: more strings than the last scan";
+        return ADM_ERR_ARG;
+    }
+    ADM_HIP(hipMemcpyAsync(c.h_ctr, c.d_ctr, kCtrWords * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
+    if (n > 0) {
+        ADM_HIP(hipMemcpyAsync(c.h_res_count, c.d_res_count, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c.stream));
+        ADM_HIP(hipMemcpyAsync(c.h_res_start, c.d_res_start, sizeof(int64_t) * n, hipMemcpyDeviceToHost, c.stream));
+    }
+    ADM_HIP(hipEventRecord(c.done, c.stream));
+    return ADM_OK;
+}
+
+int Engine::finish_collect(Context& c, int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids,
+                           std::string& err) {
+    ADM_HIP(hipEventSynchronize(c.done));
     float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ev0_, ev1_) == hipSuccess) stats_.kernel_ms = ms;
+    if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) c.stats.kernel_ms = ms;
     unsigned long long top = 0;
-    std::memcpy(&top, h_ctr_ + kCtrPoolTop, 8);
-    std::memcpy(stats_.method_steps, h_ctr_ + kCtrMethods, sizeof(stats_.method_steps));
-    stats_.pool_used = (long long)top;
-    if (h_ctr_[kCtrOverflow]) {
+    std::memcpy(&top, c.h_ctr + kCtrPoolTop, 8);
+    std::memcpy(c.stats.method_steps, c.h_ctr + kCtrMethods, sizeof(c.stats.method_steps));
+    c.stats.pool_used = (long long)top;
+    if (c.h_ctr[kCtrOverflow]) {
         err = "accepted-state pool overflow";
         return ADM_ERR_CAPACITY;
     }
-    std::vector<uint32_t> pool(size_t(std::min<unsigned long long>(top, (unsigned long long)pool_cap_)));
+    std::vector<uint32_t> pool(size_t(std::min<unsigned long long>(top, (unsigned long long)c.pool_cap)));
     if (!pool.empty()) {
-        ADM_HIP(hipMemcpyAsync(pool.data(), d_pool_, sizeof(uint32_t) * pool.size(), hipMemcpyDeviceToHost, st));
-        ADM_HIP(hipStreamSynchronize(st));
+        ADM_HIP(hipMemcpyAsync(pool.data(), c.d_pool, sizeof(uint32_t) * pool.size(), hipMemcpyDeviceToHost,
+                               c.stream));
+        ADM_HIP(hipStreamSynchronize(c.stream));
     }
     // accept map lookup, sort, unique (src/iNFA_kernel.cu:290-307)
+    out_off.assign(size_t(n) + 1, 0);
+    out_ids.clear();
     std::vector<int32_t> tmp;
     for (int32_t i = 0; i < n; ++i) {
         tmp.clear();
-        for (int32_t k = 0; k < cnt[i]; ++k) {
-            const uint32_t s = pool[size_t(start[i] + k)];
+        for (int32_t k = 0; k < c.h_res_count[i]; ++k) {
+            const uint32_t s = pool[size_t(c.h_res_start[i] + k)];
             for (int32_t r = g_.acc_off[s]; r < g_.acc_off[s + 1]; ++r) tmp.push_back(g_.acc_rules[r]);
         }
         std::sort(tmp.begin(), tmp.end());
@@ -319,67 +444,24 @@ int Engine::collect(int32_t n, std::vector<int32_t>& out_off, std::vector<int32_
     return ADM_OK;
 }
 
+int Engine::collect(Context& c, int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids,
+                    std::string& err) {
+    if (int rc = begin_collect(c, n, err)) return rc;
+    return finish_collect(c, n, out_off, out_ids, err);
+}
+
 int Engine::scan_host(const uint8_t* const* text, const int32_t* len, int32_t n, KernelKind kind, int32_t mode,
                       std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err) {
-    if (n < 0 || (n > 0 && (!text || !len))) {
-        err = "bad arguments";
-        return ADM_ERR_ARG;
-    }
     if (!dev_ready_) {
         err = "rule set not on the device (allocScratch first)";
         return ADM_ERR_STATE;
     }
-    int64_t total = 0;
-    for (int32_t i = 0; i < n; ++i) {
-        if (len[i] < 0 || (len[i] > 0 && !text[i])) {
-            err = "bad string";
-            return ADM_ERR_ARG;
-        }
-        total += len[i];
-    }
-    // pinned staging grows geometrically, device staging likewise
-    if (total > h_text_cap_ || total > d_text_cap_) {
-        const int64_t cap = std::max<int64_t>(total + (total >> 1), 1 << 20);
-        drop_graphs();
-        if (h_text_) (void)hipHostFree(h_text_);
-        if (d_text_) (void)hipFree(d_text_);
-        h_text_ = nullptr;
-        d_text_ = nullptr;
-        ADM_HIP(hipHostMalloc(&h_text_, size_t(cap)));
-        ADM_HIP(hipMalloc(&d_text_, size_t(cap)));
-        h_text_cap_ = d_text_cap_ = cap;
-    }
-    if (n + 1 > off_cap_) {
-        const int32_t cap = std::max<int32_t>(n + 1 + (n >> 1), 4096);
-        drop_graphs();
-        if (h_off_) (void)hipHostFree(h_off_);
-        if (d_off_) (void)hipFree(d_off_);
-        h_off_ = nullptr;
-        d_off_ = nullptr;
-        ADM_HIP(hipHostMalloc(&h_off_, sizeof(int64_t) * cap));
-        ADM_HIP(hipMalloc(&d_off_, sizeof(int64_t) * cap));
-        off_cap_ = cap;
-    }
-    int64_t o = 0;
-    for (int32_t i = 0; i < n; ++i) {
-        h_off_[i] = o;
-        if (len[i] > 0) std::memcpy(h_text_ + o, text[i], size_t(len[i]));
-        o += len[i];
-    }
-    h_off_[n] = o;
-    ADM_HIP(hipMemcpyAsync(d_text_, h_text_, size_t(std::max<int64_t>(total, 1)), hipMemcpyHostToDevice, stream_));
-    ADM_HIP(hipMemcpyAsync(d_off_, h_off_, sizeof(int64_t) * (size_t(n) + 1), hipMemcpyHostToDevice, stream_));
+    if (int rc = stage(ctx_, text, len, n, err)) return rc;
     for (int attempt = 0; attempt < 4; ++attempt) {
-        if (int rc = scan_device(d_text_, d_off_, n, kind, mode, stream_, false, err)) return rc;
-        int rc = collect(n, out_off, out_ids, stream_, err);
+        if (int rc = scan_device(ctx_, ctx_.d_text, ctx_.d_off, n, kind, mode, false, err)) return rc;
+        int rc = collect(ctx_, n, out_off, out_ids, err);
         if (rc != ADM_ERR_CAPACITY) return rc;
-        // pool overflow: grow and rescan
-        const int64_t want = pool_cap_ * 4;
-        drop_graphs();
-        if (d_pool_) (void)hipFree(d_pool_);
-        d_pool_ = nullptr;
-        ADM_HIP(hipMalloc(&d_pool_, sizeof(uint32_t) * want));
-        pool_cap_ = want;
+        if (int rg = ctx_.grow_pool(err)) return rg;  // pool overflow: grow and rescan
     }
     err = "accepted-state pool overflow";
     return ADM_ERR_CAPACITY;
@@ -390,23 +472,18 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
         err = "rule set not on the device";
         return ADM_ERR_STATE;
     }
+    drop_all_graphs();
     if (w == nullptr) {
-        if (d_mlp_) (void)hipFree(d_mlp_);
-        d_mlp_ = nullptr;
-        drop_graphs();
+        dfree(d_mlp_);
         return ADM_OK;
     }
     if (n != kMlpParams) {
         err = "selector weights must hold " + std::to_string(kMlpParams) + " floats";
         return ADM_ERR_ARG;
     }
-    if (!d_mlp_) {
-        drop_graphs();
-        ADM_HIP(hipMalloc(&d_mlp_, sizeof(float) * kMlpParams));
-    }
+    if (!d_mlp_) ADM_HIP(hipMalloc(&d_mlp_, sizeof(float) * kMlpParams));
     ADM_HIP(hipMemcpy(d_mlp_, w, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
     mlp_every_ = every >= 0 ? every : 0;
-    drop_graphs();
     return ADM_OK;
 }
 
@@ -415,9 +492,9 @@ int Engine::set_log(int64_t cap_records, int32_t stride, std::string& err) {
         err = "rule set not on the device";
         return ADM_ERR_STATE;
     }
-    drop_graphs();
-    if (d_log_) (void)hipFree(d_log_);
-    d_log_ = nullptr;
+    drop_all_graphs();
+    if (ctx_.stream) ADM_HIP(hipStreamSynchronize(ctx_.stream));
+    dfree(d_log_);
     log_cap_ = 0;
     if (cap_records <= 0) return ADM_OK;
     ADM_HIP(hipMalloc(&d_log_, sizeof(float) * kLogRecord * size_t(cap_records)));
@@ -428,10 +505,10 @@ int Engine::set_log(int64_t cap_records, int32_t stride, std::string& err) {
 
 int64_t Engine::fetch_log(float* out, int64_t cap_records, std::string& err) {
     if (!d_log_) return 0;
-    ADM_HIP(hipStreamSynchronize(stream_));
-    ADM_HIP(hipMemcpy(h_ctr_, d_ctr_, kCtrWords * sizeof(int32_t), hipMemcpyDeviceToHost));
+    ADM_HIP(hipStreamSynchronize(ctx_.stream));
+    ADM_HIP(hipMemcpy(ctx_.h_ctr, ctx_.d_ctr, kCtrWords * sizeof(int32_t), hipMemcpyDeviceToHost));
     unsigned long long top = 0;
-    std::memcpy(&top, h_ctr_ + kCtrLogTop, 8);
+    std::memcpy(&top, ctx_.h_ctr + kCtrLogTop, 8);
     int64_t n = std::min<int64_t>(int64_t(top), log_cap_);
     n = std::min<int64_t>(n, cap_records);
     if (n > 0 && out) ADM_HIP(hipMemcpy(out, d_log_, sizeof(float) * kLogRecord * size_t(n), hipMemcpyDeviceToHost));

@@ -1,11 +1,13 @@
 // Host side of the scan engine: device residency of a compiled rule set,
-// staging buffers, launches, hipGraph replay and result extraction.
+// per-stream scan contexts (buffers, staging, captured graphs), launches and
+// result extraction.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -37,9 +39,45 @@ struct Variant {
 };
 
 struct ScanStats {
-    float kernel_ms = 0.f;            // last scan: hipEvent time of the scan kernel
+    float kernel_ms = 0.f;  // last scan: hipEvent time of the launch sequence
     unsigned long long method_steps[3] = {0, 0, 0};
     long long pool_used = 0;
+};
+
+// Everything one in-flight scan needs besides the rule set: a stream,
+// result buffers, counters, host/device staging and captured graphs.
+// Contexts of one engine may run concurrently on the device.
+struct Context {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+    int32_t res_n_cap = 0;
+    int64_t pool_cap = 0;
+    int32_t* d_res_count = nullptr;
+    int64_t* d_res_start = nullptr;
+    uint32_t* d_pool = nullptr;
+    int32_t* d_order = nullptr;
+    int32_t* d_ctr = nullptr;
+    int32_t* h_ctr = nullptr;  // pinned mirrors for asynchronous collection
+    int32_t* h_res_count = nullptr;
+    int64_t* h_res_start = nullptr;
+    int32_t h_res_cap = 0;
+    uint8_t* h_text = nullptr;
+    uint8_t* d_text = nullptr;
+    int64_t text_cap = 0;
+    int64_t* h_off = nullptr;
+    int64_t* d_off = nullptr;
+    int32_t off_cap = 0;
+    std::map<std::tuple<const void*, const void*, int32_t, int32_t, int32_t>, hipGraphExec_t> graphs;
+    ScanStats stats;
+
+    ~Context();
+    int init(std::string& err);
+    void release();
+    void drop_graphs();
+    int ensure_results(int32_t n, std::string& err);
+    int ensure_staging(int64_t bytes, int32_t n, std::string& err);
+    int grow_pool(std::string& err);
+    size_t device_bytes() const;
 };
 
 class Engine {
@@ -55,79 +93,74 @@ class Engine {
     bool on_device() const { return dev_ready_; }
     const Graph& graph() const { return g_; }
 
-    // Scans n strings already resident in device memory (text + offsets[n+1]).
-    // Results stay on the device until collect().
+    // --- default context (the handle's own stream) ---
     int scan_device(const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
-                    hipStream_t stream, bool use_graph, std::string& err);
-    // Copies the results of the last scan_device to the host and maps
-    // accepted states to rule ids: ids of string i are
-    // out_ids[out_off[i] .. out_off[i+1]), sorted and unique.
-    int collect(int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, hipStream_t stream,
-                std::string& err);
-    // Host strings: stage, scan, collect (the BatchedScan path).
+                    bool use_graph, std::string& err) {
+        return scan_device(ctx_, d_text, d_off, n, kind, mode, use_graph, err);
+    }
+    int collect(int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err) {
+        return collect(ctx_, n, out_off, out_ids, err);
+    }
     int scan_host(const uint8_t* const* text, const int32_t* len, int32_t n, KernelKind kind, int32_t mode,
                   std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err);
+    const ScanStats& stats() const { return ctx_.stats; }
+    Context& default_context() { return ctx_; }
 
-    void set_variant(const Variant& v) {
-        var_ = v;
-        drop_graphs();
-    }
+    // --- any context ---
+    // Scans n strings already in device memory (text + offsets[n+1]) on the
+    // context's stream; results stay on the device until collected.
+    int scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind,
+                    int32_t mode, bool use_graph, std::string& err);
+    // Copies host strings into the context's pinned staging and enqueues
+    // the host-to-device copy.
+    int stage(Context& c, const uint8_t* const* text, const int32_t* len, int32_t n, std::string& err);
+    // Enqueues the result copies and records c.done (no host wait).
+    int begin_collect(Context& c, int32_t n, std::string& err);
+    // After c.done: maps accepted states to sorted unique rule ids, string i
+    // in out_ids[out_off[i] .. out_off[i+1]).
+    int finish_collect(Context& c, int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids,
+                       std::string& err);
+    int collect(Context& c, int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids,
+                std::string& err);
+    int new_context(std::unique_ptr<Context>& out, std::string& err);
+    void forget_context(Context* c);
+
+    void set_variant(const Variant& v);
     const Variant& variant() const { return var_; }
     int set_mlp(const float* w, int32_t n, int32_t every, std::string& err);
     bool has_mlp() const { return d_mlp_ != nullptr; }
     // step logging for the learned selector (MODE_LOG)
     int set_log(int64_t cap_records, int32_t stride, std::string& err);
     int64_t fetch_log(float* out, int64_t cap_records, std::string& err);
-    const ScanStats& stats() const { return stats_; }
-    hipStream_t stream() const { return stream_; }
     size_t engine_lds_bytes(const Variant& v) const;
     size_t plain_lds_bytes() const;
     int device_cu_count() const { return cu_count_; }
+    size_t table_bytes() const { return table_bytes_; }
 
    private:
-    int ensure_results(int32_t n, std::string& err);
-    int launch(const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
-               hipStream_t stream, std::string& err);
-    ScanArgs make_args(const uint8_t* d_text, const int64_t* d_off, int32_t n, int32_t mode);
+    int launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
+               std::string& err);
+    ScanArgs make_args(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, int32_t mode);
+    void drop_all_graphs();
 
     Graph g_;
     bool loaded_ = false, dev_ready_ = false;
     DevGraph dev_{};
     std::vector<void*> dev_allocs_;
-    hipStream_t stream_ = nullptr;
-    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    size_t table_bytes_ = 0;
     int cu_count_ = 0;
     size_t max_lds_ = 0;
     Variant var_;
-    // result buffers
-    int32_t res_n_cap_ = 0;
-    int64_t pool_cap_ = 0;
-    int32_t* d_res_count_ = nullptr;
-    int64_t* d_res_start_ = nullptr;
-    uint32_t* d_pool_ = nullptr;
-    int32_t* d_order_ = nullptr;
-    int32_t* d_ctr_ = nullptr;  // counters (see kCtr*)
-    int32_t* h_ctr_ = nullptr;  // pinned mirror
     float* d_mlp_ = nullptr;
     int32_t mlp_every_ = 0;  // 0: per-wave decision cache
     float* d_log_ = nullptr;
     int64_t log_cap_ = 0;
     int32_t log_stride_ = 64;
-    // host staging for scan_host
-    uint8_t* h_text_ = nullptr;
-    int64_t h_text_cap_ = 0;
-    uint8_t* d_text_ = nullptr;
-    int64_t d_text_cap_ = 0;
-    int64_t* h_off_ = nullptr;
-    int64_t* d_off_ = nullptr;
-    int32_t off_cap_ = 0;
-    ScanStats stats_;
-    // captured scans keyed by (text, offsets, n, kind, mode, stream)
-    std::map<std::tuple<const void*, const void*, int32_t, int32_t, int32_t, void*>, hipGraphExec_t> graphs_;
-    void drop_graphs();
+    Context ctx_;
+    std::vector<Context*> contexts_;  // every context handed out by new_context
 };
 
-// counters layout in d_ctr_ (int32 words)
+// counters layout in Context::d_ctr (int32 words)
 constexpr int kCtrQueue = 0;
 constexpr int kCtrOverflow = 1;
 constexpr int kCtrPoolTop = 2;    // u64

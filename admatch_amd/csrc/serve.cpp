@@ -1,0 +1,174 @@
+// Continuous-batching request loop over one resident rule set.
+//
+// Requests (strings with arrival times) are replayed in real time.  The loop
+// keeps `lanes` scan contexts, each with its own HIP stream, staging and
+// result buffers; a lane that is free takes every request that has arrived
+// and not started yet (up to max_batch strings / max_batch_bytes, and as
+// long as the device bytes of all in-flight batches stay within the state
+// budget), copies it in, launches the scan and queues the result copies
+// without waiting.  Lanes run concurrently on the device, so requests that
+// arrive while others are being scanned start as soon as a lane and budget
+// free up.  Completion is detected by polling each lane's event; a request's
+// latency is its completion time minus its arrival time.
+#include "serve.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <thread>
+
+namespace admatch {
+
+namespace {
+using clk = std::chrono::steady_clock;
+
+struct Lane {
+    std::unique_ptr<Context> ctx;
+    bool busy = false;
+    std::vector<int32_t> ids;  // request ids of the batch
+    int64_t device_bytes = 0;
+};
+
+// device bytes a batch occupies while in flight: input bytes, offsets,
+// order/count/start per string, accepted-state pool
+int64_t batch_bytes(int64_t text, int32_t n) { return text + 8 * int64_t(n + 1) + 16 * int64_t(n) + 32 * int64_t(n); }
+}  // namespace
+
+int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const int64_t* offsets,
+                const double* arrival_s, int32_t n, ServeResult& out, std::string& err) {
+    if (n < 0 || (n > 0 && (!buf || !offsets || !arrival_s)) || cfg.lanes < 1 || cfg.max_batch < 1) {
+        err = "bad arguments";
+        return ADM_ERR_ARG;
+    }
+    for (int32_t i = 1; i < n; ++i)
+        if (arrival_s[i] < arrival_s[i - 1]) {
+            err = "arrival times must be non-decreasing";
+            return ADM_ERR_ARG;
+        }
+    int64_t max_req = 0;
+    for (int32_t i = 0; i < n; ++i) max_req = std::max<int64_t>(max_req, offsets[i + 1] - offsets[i]);
+    if (batch_bytes(max_req, 1) > cfg.state_budget_bytes) {
+        err = "a single request exceeds the state budget";
+        return ADM_ERR_BUDGET;
+    }
+    std::vector<Lane> lanes(size_t(cfg.lanes));
+    for (auto& l : lanes) {
+        if (int rc = eng.new_context(l.ctx, err)) return rc;
+        // pre-size buffers so the loop does not allocate in steady state
+        if (int rc = l.ctx->ensure_staging(std::max<int64_t>(cfg.max_batch_bytes, max_req), cfg.max_batch, err))
+            return rc;
+        if (int rc = l.ctx->ensure_results(cfg.max_batch, err)) return rc;
+    }
+    struct Guard {
+        Engine& e;
+        std::vector<Lane>& l;
+        ~Guard() {
+            for (auto& x : l)
+                if (x.ctx) e.forget_context(x.ctx.get());
+        }
+    } guard{eng, lanes};
+
+    out.latency_s.assign(size_t(n), 0.0);
+    std::vector<std::vector<int32_t>> res(static_cast<size_t>(n));
+    std::deque<int32_t> pending;
+    int32_t next = 0, completed = 0;
+    int64_t inflight = 0;
+    out.batches = 0;
+    out.max_inflight_bytes = 0;
+    out.max_lanes_busy = 0;
+    double batch_sum = 0;
+    std::vector<const uint8_t*> ptrs;
+    std::vector<int32_t> lens;
+    std::vector<int32_t> boff, bids;
+    const auto t0 = clk::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count(); };
+
+    while (completed < n) {
+        bool progressed = false;
+        const double t = now_s();
+        while (next < n && arrival_s[next] <= t) pending.push_back(next++);
+        // completions
+        for (auto& l : lanes) {
+            if (!l.busy) continue;
+            const hipError_t q = hipEventQuery(l.ctx->done);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                err = std::string("hipEventQuery: ") + hipGetErrorString(q);
+                return ADM_ERR_HIP;
+            }
+            const int32_t bn = int32_t(l.ids.size());
+            if (int rc = eng.finish_collect(*l.ctx, bn, boff, bids, err)) return rc;
+            const double tdone = now_s();
+            for (int32_t k = 0; k < bn; ++k) {
+                const int32_t r = l.ids[size_t(k)];
+                res[size_t(r)].assign(bids.begin() + boff[k], bids.begin() + boff[k + 1]);
+                out.latency_s[size_t(r)] = tdone - arrival_s[r];
+            }
+            completed += bn;
+            inflight -= l.device_bytes;
+            l.busy = false;
+            l.ids.clear();
+            progressed = true;
+        }
+        // admission
+        for (auto& l : lanes) {
+            if (l.busy || pending.empty()) continue;
+            int64_t text = 0;
+            l.ids.clear();
+            while (!pending.empty() && int32_t(l.ids.size()) < cfg.max_batch) {
+                const int32_t r = pending.front();
+                const int64_t len = offsets[r + 1] - offsets[r];
+                if (!l.ids.empty() && text + len > cfg.max_batch_bytes) break;
+                if (inflight + batch_bytes(text + len, int32_t(l.ids.size()) + 1) > cfg.state_budget_bytes) break;
+                l.ids.push_back(r);
+                text += len;
+                pending.pop_front();
+            }
+            if (l.ids.empty()) break;  // budget exhausted until a lane completes
+            const int32_t bn = int32_t(l.ids.size());
+            ptrs.resize(size_t(bn));
+            lens.resize(size_t(bn));
+            for (int32_t k = 0; k < bn; ++k) {
+                const int32_t r = l.ids[size_t(k)];
+                ptrs[size_t(k)] = buf + offsets[r];
+                lens[size_t(k)] = int32_t(offsets[r + 1] - offsets[r]);
+            }
+            if (int rc = eng.stage(*l.ctx, ptrs.data(), lens.data(), bn, err)) return rc;
+            if (int rc = eng.scan_device(*l.ctx, l.ctx->d_text, l.ctx->d_off, bn, KernelKind(cfg.kind), cfg.mode,
+                                         false, err))
+                return rc;
+            if (int rc = eng.begin_collect(*l.ctx, bn, err)) return rc;
+            l.device_bytes = batch_bytes(text, bn);
+            inflight += l.device_bytes;
+            l.busy = true;
+            out.batches += 1;
+            batch_sum += bn;
+            out.max_inflight_bytes = std::max(out.max_inflight_bytes, inflight);
+            progressed = true;
+        }
+        int busy = 0;
+        for (auto& l : lanes) busy += l.busy;
+        out.max_lanes_busy = std::max(out.max_lanes_busy, busy);
+        if (!progressed) {
+            if (busy == 0 && pending.empty() && next < n) {
+                // idle until the next arrival
+                const double wait = arrival_s[next] - now_s();
+                if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
+            } else {
+                std::this_thread::yield();
+            }
+        }
+    }
+    out.wall_s = now_s();
+    out.mean_batch = out.batches ? batch_sum / double(out.batches) : 0.0;
+    out.offsets.assign(size_t(n) + 1, 0);
+    out.ids.clear();
+    for (int32_t i = 0; i < n; ++i) {
+        out.ids.insert(out.ids.end(), res[size_t(i)].begin(), res[size_t(i)].end());
+        out.offsets[size_t(i) + 1] = int32_t(out.ids.size());
+    }
+    return ADM_OK;
+}
+
+}  // namespace admatch

@@ -42,7 +42,7 @@ EXPORTED = [
     "ita_create", "ita_alloc", "ita_free", "ita_destroy", "ita_scan", "ita_batched_scan", "ita_scan_device",
     "ita_collect", "ita_last_stats", "ita_set_variant", "ita_get_variant", "ita_set_selector", "ita_set_log",
     "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
-    "ita_selector_forward",
+    "ita_selector_forward", "ita_serve_trace",
 ]
 
 _LIB = None

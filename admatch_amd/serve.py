@@ -1,0 +1,107 @@
+"""Serving loop driver: Poisson request traces replayed through the native
+continuous-batching loop (ita_serve_trace, admatch_amd/csrc/serve.cpp).
+
+    python -m admatch_amd.serve --rates 50000,200000,800000 --requests 40000
+
+Requests are strings of the serving workload (the large 3,000-rule set,
+lognormal lengths with median 800 B, max 16 KB).  For each arrival rate the
+learned-selector engine and the plain iNFA path are run on the same trace;
+reported per rate: p50/p99 latency, achieved request and byte throughput,
+and the batching the loop formed.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+from . import autotune, engine as E, selector, synth
+
+
+def _sig():
+    L = E.lib()
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    L.ita_serve_trace.restype = i32
+    L.ita_serve_trace.argtypes = [vp, i32, i32, i32, i32, i64, i64, vp, vp, vp, i32, vp, vp, vp, i64, vp]
+    return L
+
+
+def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    return np.cumsum(rng.exponential(1.0 / rate, size=n))
+
+
+def serve_trace(s: E.Scratch, buf: np.ndarray, offsets: np.ndarray, arrival_s: np.ndarray, kind: int = E.KIND_ENGINE,
+                mode: int = E.MODE_LEARNED, lanes: int = 4, max_batch: int = 4096, max_batch_bytes: int = 8 << 20,
+                state_budget: int = 256 << 20):
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    arrival_s = np.ascontiguousarray(arrival_s, dtype=np.float64)
+    n = offsets.size - 1
+    lat = np.zeros(max(n, 1), dtype=np.float64)
+    out_off = np.zeros(n + 1, dtype=np.int32)
+    stats = np.zeros(6, dtype=np.float64)
+    cap = max(4 * n, 1024)
+    while True:
+        ids = np.zeros(cap, dtype=np.int32)
+        rc = _sig().ita_serve_trace(s._h, kind, mode, lanes, max_batch, max_batch_bytes, state_budget,
+                                    buf.ctypes.data, offsets.ctypes.data, arrival_s.ctypes.data, n,
+                                    lat.ctypes.data, out_off.ctypes.data, ids.ctypes.data, cap, stats.ctypes.data)
+        if rc == E.ITA_ERR_CAPACITY:
+            cap = int(out_off[-1])
+            continue
+        E._err(rc)
+        break
+    info = {"wall_s": stats[0], "batches": int(stats[1]), "mean_batch": stats[2],
+            "peak_inflight_bytes": int(stats[3]), "peak_lanes_busy": int(stats[4]), "bytes": int(stats[5])}
+    return lat[:n], E.unpack(out_off, ids[: out_off[-1]]), info
+
+
+def summarize(lat: np.ndarray, info: dict, n: int) -> dict:
+    return {"p50_us": float(np.percentile(lat, 50) * 1e6), "p99_us": float(np.percentile(lat, 99) * 1e6),
+            "mean_us": float(lat.mean() * 1e6), "req_per_s": n / info["wall_s"],
+            "gbs": info["bytes"] / info["wall_s"] / 1e9, "batches": info["batches"],
+            "mean_batch": round(info["mean_batch"], 1), "peak_lanes_busy": info["peak_lanes_busy"]}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--rates", default="50000,200000,800000")
+    ap.add_argument("--requests", type=int, default=40000)
+    ap.add_argument("--lanes", type=int, default=4)
+    ap.add_argument("--plain-requests", type=int, default=4000)
+    args = ap.parse_args(argv)
+    w = synth.make_workload("serve", n=args.requests)
+    d = tempfile.mkdtemp()
+    path = os.path.join(d, "serve.nfa")
+    w.ruleset.write(path)
+    s = E.Scratch(E.AD_MATCHER | E.LEARNED_SELECTOR, path).alloc_scratch()
+    autotune.apply(s)
+    wts = selector.load_weights()
+    if wts is not None:
+        s.set_selector(wts)
+    report = {"workload": f"{w.ruleset.state_count} states, {w.inputs.n} requests, "
+                          f"{int(w.inputs.offsets[-1])} B", "learned_selector": wts is not None, "rates": {}}
+    for rate in (float(r) for r in args.rates.split(",")):
+        arr = poisson_arrivals(w.inputs.n, rate, seed=int(rate))
+        lat, res, info = serve_trace(s, w.inputs.buf, w.inputs.offsets, arr, mode=E.MODE_LEARNED,
+                                     lanes=args.lanes)
+        eng = summarize(lat, info, w.inputs.n)
+        # plain iNFA on a prefix of the same trace
+        k = min(args.plain_requests, w.inputs.n)
+        lat2, res2, info2 = serve_trace(s, w.inputs.buf, w.inputs.offsets[: k + 1], arr[:k], kind=E.KIND_PLAIN,
+                                        mode=E.MODE_INFA, lanes=args.lanes)
+        assert res2 == res[:k], "plain iNFA and the engine disagree"
+        report["rates"][str(int(rate))] = {"engine": eng, "plain_infa": summarize(lat2, info2, k)}
+        print(f"[serve] rate {rate:.0f}/s engine {eng}", file=sys.stderr, flush=True)
+    print(json.dumps(report))
+    s.close()
+
+
+if __name__ == "__main__":
+    main()
