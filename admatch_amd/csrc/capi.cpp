@@ -273,6 +273,10 @@ int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap) {
             v.insert(v.end(), g.accept.begin(), g.accept.end());
             return give(v.data(), int64_t(v.size()), 8);
         }
+        case 10: return give(g.as_tab.data(), int64_t(g.as_tab.size()), 4);
+        case 11: return give(g.as_multi.data(), int64_t(g.as_multi.size()), 4);
+        case 12: return give(g.alw_off.data(), int64_t(g.alw_off.size()), 4);
+        case 13: return give(g.alw_dst.data(), int64_t(g.alw_dst.size()), 4);
         default: return fail(ITA_ERR_ARG, "bad table id");
     }
 }

@@ -19,9 +19,11 @@ struct DevGraph {
     const uint64_t* tko_lim;  // [K*W]
     const int32_t* tko_loff;  // [258]
     const uint2* tko_left;    // leftover (src,dst)
-    const uint32_t* as_off;   // [S*256+1]
-    const uint32_t* as_dst;
-    const int4* sym_stats;    // [256]: |T_c|, k_c, |leftover_c|, 0
+    const uint32_t* as_tab;   // [256*S] AS rows, byte major (null: AS unavailable)
+    const uint32_t* as_multi; // destinations of multi-entry rows
+    const uint32_t* alw_dst;  // always-active rows per byte
+    int32_t alw_total;        // alw_dst entries
+    const int4* sym_stats;    // [256]: |T_c|, k_c, |leftover_c|, start in alw_dst
     int32_t wb_begin, wb_count;  // word-boundary slice of list
 };
 

@@ -237,12 +237,19 @@ int Engine::upload(std::string& err) {
     ADM_PUT(g_.tko_lim, tko_lim)
     ADM_PUT(g_.tko_loff, tko_loff)
     ADM_PUT(g_.tko_left, tko_left)
-    ADM_PUT(g_.as_off, as_off)
-    ADM_PUT(g_.as_dst, as_dst)
+    if (g_.as_ok) {
+        ADM_PUT(g_.as_tab, as_tab)
+        ADM_PUT(g_.as_multi, as_multi)
+    } else {
+        dev_.as_tab = nullptr;  // the AS method falls back to iNFA
+        dev_.as_multi = nullptr;
+    }
+    ADM_PUT(g_.alw_dst, alw_dst)
+    dev_.alw_total = g_.alw_off[256];
     std::vector<int4> st(256);
     for (int c = 0; c < 256; ++c)
         st[c] = make_int4(g_.sym_off[c + 1] - g_.sym_off[c], g_.tko_koff[c + 1] - g_.tko_koff[c],
-                          g_.tko_loff[c + 1] - g_.tko_loff[c], 0);
+                          g_.tko_loff[c + 1] - g_.tko_loff[c], g_.alw_off[c]);
     ADM_PUT(st, sym_stats)
 #undef ADM_PUT
     dev_.wb_begin = g_.sym_off[kWordBoundary];

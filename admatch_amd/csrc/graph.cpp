@@ -338,6 +338,66 @@ void build_as(Graph& g) {
             const Edge& e = g.list[t];
             g.as_dst[fill[size_t(e.src) * 256 + c]++] = e.dst;
         }
+
+    // device form: rows plus the persistent self-loop
+    auto persistent = [&](uint32_t s) { return (g.persis[s >> 6] >> (s & 63)) & 1ull; };
+    auto always = [&](uint32_t s) { return persistent(s) && ((g.init[s >> 6] >> (s & 63)) & 1ull); };
+    g.as_tab.assign(size_t(g.S) * 256, 0);
+    g.as_multi.clear();
+    g.as_ok = true;
+    std::vector<uint32_t> row;
+    for (int32_t s = 0; s < g.S && g.as_ok; ++s) {
+        const bool p = persistent(uint32_t(s));
+        for (int c = 0; c < 256; ++c) {
+            const uint32_t* b = g.as_dst.data() + g.as_off[size_t(s) * 256 + c];
+            const uint32_t* e = g.as_dst.data() + g.as_off[size_t(s) * 256 + c + 1];
+            row.assign(b, e);
+            if (p && !std::binary_search(row.begin(), row.end(), uint32_t(s)))
+                row.insert(std::lower_bound(row.begin(), row.end(), uint32_t(s)), uint32_t(s));
+            uint32_t& ent = g.as_tab[size_t(c) * g.S + s];
+            if (row.empty()) continue;
+            if (row.size() == 1) {
+                ent = (1u << 24) | row[0];
+                continue;
+            }
+            const size_t idx = g.as_multi.size();
+            if (idx + row.size() + 1 >= (size_t(1) << 24)) {
+                g.as_ok = false;
+                break;
+            }
+            if (row.size() >= 255) {
+                g.as_multi.push_back(uint32_t(row.size()));
+                ent = (255u << 24) | uint32_t(idx);
+            } else {
+                ent = (uint32_t(row.size()) << 24) | uint32_t(idx);
+            }
+            g.as_multi.insert(g.as_multi.end(), row.begin(), row.end());
+        }
+    }
+    if (!g.as_ok) {
+        g.as_tab.clear();
+        g.as_multi.clear();
+    }
+    // rows of the always-active states, merged per byte
+    g.alw_off.assign(257, 0);
+    g.alw_dst.clear();
+    std::vector<uint32_t> alw;
+    for (int32_t w = 0; w < g.W; ++w)
+        for (uint64_t v = g.persis[w] & g.init[w]; v; v &= v - 1) alw.push_back(uint32_t(w * 64 + __builtin_ctzll(v)));
+    for (int c = 0; c < 256; ++c) {
+        g.alw_off[c] = int32_t(g.alw_dst.size());
+        row.clear();
+        for (uint32_t s : alw) {
+            row.push_back(s);
+            const uint32_t* b = g.as_dst.data() + g.as_off[size_t(s) * 256 + c];
+            const uint32_t* e = g.as_dst.data() + g.as_off[size_t(s) * 256 + c + 1];
+            row.insert(row.end(), b, e);
+        }
+        std::sort(row.begin(), row.end());
+        row.erase(std::unique(row.begin(), row.end()), row.end());
+        for (uint32_t d : row) g.alw_dst.push_back(always(d) ? (d | 0x80000000u) : d);
+    }
+    g.alw_off[256] = int32_t(g.alw_dst.size());
 }
 
 }  // namespace admatch

@@ -22,7 +22,7 @@ namespace admatch {
 constexpr int kSymbols = 257;      // 256 byte values + word boundary (src/common.h:13)
 constexpr int kWordBoundary = 256;  // src/common.h:15
 constexpr int kMaxK = 1024;        // src/common.h:21
-constexpr int32_t kMaxStates = 1 << 26;
+constexpr int32_t kMaxStates = 1 << 24;  // ids fit the 24-bit payload of the AS table
 
 struct Edge {
     uint32_t src, dst;
@@ -57,6 +57,21 @@ struct Graph {
     // mask instead).
     std::vector<uint32_t> as_off, as_dst;
 
+    // Device form of the AS rows, one 32-bit entry per (byte, state), byte
+    // major (as_tab[c*S + s]): count in the top 8 bits (0 = empty, 255 =
+    // long row whose count is as_multi[idx]), then either the single
+    // destination (count 1) or the row's index into as_multi.  Rows of
+    // persistent states include the state itself, so the AS step needs no
+    // separate persistent mask.
+    std::vector<uint32_t> as_tab, as_multi;
+    bool as_ok = false;  // false when the rows do not fit the 24-bit index
+    // Always-active states (initial and persistent: active at every step).
+    // Their combined rows per byte, ascending, with bit 31 marking a
+    // destination that is itself always active: alw_dst[alw_off[c] ..
+    // alw_off[c+1]).
+    std::vector<int32_t> alw_off;
+    std::vector<uint32_t> alw_dst;
+
     // parse statistics
     int64_t n_persis_lines = 0, n_bad_lines = 0, n_raw_transitions = 0;
 
@@ -71,7 +86,7 @@ int load_nfa(const char* path, Graph& g, std::string& err);
 // TKO top-k partition; threshold < 0 selects the reference's S/16.
 void build_tko(Graph& g, int32_t threshold = -1);
 
-// Sparse active-state rows from the iNFA lists.
+// Sparse active-state rows from the iNFA lists, and their device form.
 void build_as(Graph& g);
 
 }  // namespace admatch

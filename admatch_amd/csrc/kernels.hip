@@ -21,6 +21,11 @@
 namespace admatch {
 
 // ------------------------------------------------------------------ common
+// List entries carry bit 31 for always-active states (their rows come from
+// the per-byte always-active list instead of the AS table).
+constexpr uint32_t kTag = 0x80000000u;
+constexpr uint32_t kIdMask = 0x00ffffffu;
+
 __device__ __forceinline__ bool set_new(uint64_t* bits, uint32_t d) {
     const unsigned long long m = 1ull << (d & 63);
     const unsigned long long old = atomicOr(reinterpret_cast<unsigned long long*>(&bits[d >> 6]), m);
@@ -195,7 +200,7 @@ __device__ int persist_sparse(const DevGraph& g, const uint32_t* cl, int n_cur, 
     for (int k0 = 0; k0 < n_cur; k0 += 64) {
         const int k = k0 + lane_id();
         const bool ok = k < n_cur;
-        const uint32_t s = ok ? cl[k] : 0u;
+        const uint32_t s = ok ? (cl[k] & kIdMask) : 0u;
         const bool p = ok && test_bit(g.persis, s);
         if (p) set_bit_atomic(fut, s);
         n = list_append<CAP>(fl, n, p, s);
@@ -250,39 +255,68 @@ __device__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64
     }
 }
 
-// AS rows of the active states on byte c, flattened so that each lane
-// fetches one destination per iteration (coalesced within a row).
+// Flattened processing of multi-destination rows: lane l owns a row of
+// len_l destinations starting at as_multi[start_l]; each iteration every
+// lane fetches one destination (coalesced within a row).
 template <int CAP>
-__device__ int as_rows(const DevGraph& g, int c, const uint32_t* cl, int n_cur, uint64_t* fut, uint32_t* fl, int n) {
+__device__ int multi_rows(const DevGraph& g, uint32_t start, int len, uint64_t* fut, uint32_t* fl, int n) {
+    const int incl = wave_scan(len);
+    const int excl = incl - len;
+    const int tot = uniform(__shfl(incl, 63, 64));
+    for (int t0 = 0; t0 < tot; t0 += 64) {
+        const int t = t0 + lane_id();
+        int lo = 0, hi = 63;  // owner lane: smallest r with incl[r] > t
+#pragma unroll
+        for (int it = 0; it < 6; ++it) {
+            const int mid = (lo + hi) >> 1;
+            const int v = __shfl(incl, mid, 64);
+            if (v > t) hi = mid;
+            else lo = mid + 1;
+        }
+        const uint32_t so = uint32_t(__shfl(int(start), lo, 64));
+        const int exo = __shfl(excl, lo, 64);
+        const bool ok = t < tot;
+        const uint32_t d = ok ? g.as_multi[so + uint32_t(t - exo)] : 0u;
+        const bool nw = ok && set_new(fut, d);
+        n = list_append<CAP>(fl, n, nw, d);
+    }
+    return n;
+}
+
+// AS step (src/AS_kernel.cu:74-125 with sparse rows): fut = union of the
+// rows of the active states on byte c.  Rows of persistent states contain
+// the state itself.  pre holds the first 64 always-active entries of c,
+// loaded one step ahead.
+template <int CAP>
+__device__ int as_step(const DevGraph& g, int c, int alw_b, int alw_n, uint32_t pre, const uint32_t* cl, int n_cur,
+                       uint64_t* fut, uint32_t* fl) {
+    int n = 0;
+    for (int t0 = 0; t0 < alw_n; t0 += 64) {
+        const int t = t0 + lane_id();
+        const bool ok = t < alw_n;
+        const uint32_t v = t0 == 0 ? pre : (ok ? g.alw_dst[alw_b + t] : 0u);
+        const bool nw = ok && set_new(fut, v & kIdMask);
+        n = list_append<CAP>(fl, n, nw, v);
+    }
+    const uint32_t* tab = g.as_tab + size_t(c) * g.S;
     for (int k0 = 0; k0 < n_cur; k0 += 64) {
         const int k = k0 + lane_id();
-        uint32_t rb = 0, re = 0;
-        if (k < n_cur) {
-            const uint32_t* o = g.as_off + size_t(cl[k]) * 256 + c;
-            rb = o[0];
-            re = o[1];
-        }
-        const int len = int(re - rb);
-        const int incl = wave_scan(len);
-        const int excl = incl - len;
-        const int tot = uniform(__shfl(incl, 63, 64));
-        for (int t0 = 0; t0 < tot; t0 += 64) {
-            const int t = t0 + lane_id();
-            // owner lane: smallest r with incl[r] > t
-            int lo = 0, hi = 63;
-#pragma unroll
-            for (int it = 0; it < 6; ++it) {
-                const int mid = (lo + hi) >> 1;
-                const int v = __shfl(incl, mid, 64);
-                if (v > t) hi = mid;
-                else lo = mid + 1;
+        const uint32_t sv = k < n_cur ? cl[k] : kTag;
+        const uint32_t e = (sv & kTag) ? 0u : tab[sv];
+        const uint32_t cnt = e >> 24;
+        const bool single = cnt == 1;
+        const uint32_t d = e & kIdMask;
+        const bool nw = single && set_new(fut, d);
+        n = list_append<CAP>(fl, n, nw, d);
+        const bool multi = cnt >= 2;
+        if (__ballot(multi)) {
+            uint32_t start = d;
+            int len = multi ? int(cnt) : 0;
+            if (cnt == 255) {
+                len = int(g.as_multi[d]);
+                start = d + 1;
             }
-            const uint32_t rbo = uint32_t(__shfl(int(rb), lo, 64));
-            const int exo = __shfl(excl, lo, 64);
-            const bool ok = t < tot;
-            const uint32_t d = ok ? g.as_dst[rbo + uint32_t(t - exo)] : 0u;
-            const bool nw = ok && set_new(fut, d);
-            n = list_append<CAP>(fl, n, nw, d);
+            n = multi_rows<CAP>(g, start, len, fut, fl, n);
         }
     }
     return n;
@@ -362,7 +396,7 @@ struct Lane {
 // Returns the fut count (list valid iff *fut_sparse).
 template <int CAP>
 __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, const uint32_t* cl, int n_cur,
-                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse) {
+                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse, uint32_t pre) {
     int n = 0;
     const int W = g.W;
     if (m == M_TKO) {
@@ -372,15 +406,20 @@ __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, 
         *fut_sparse = false;
         return -1;
     }
+    if (m == M_AS && sparse && g.as_tab != nullptr) {
+        const int ab = g.sym_stats[c].w;
+        const int ae = c < 255 ? g.sym_stats[c + 1].w : g.alw_total;
+        n = as_step<CAP>(g, c, ab, ae - ab, pre, cl, n_cur, fut, fl);
+        *fut_sparse = n <= CAP;
+        return n;
+    }
     if (sparse) {
         n = persist_sparse<CAP>(g, cl, n_cur, fut, fl);
     } else {
         for (int j = lane_id(); j < W; j += 64) fut[j] = cur[j] & g.persis[j];
     }
     wave_sync();
-    if (m == M_AS && sparse) {
-        n = as_rows<CAP>(g, c, cl, n_cur, fut, fl, n);
-    } else if (sparse) {
+    if (sparse) {
         n = scatter<CAP, true>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, n);
     } else {
         scatter<CAP, false>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, 0);
@@ -393,7 +432,7 @@ __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, 
 template <int CAP>
 __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list, int n, bool sparse, int W) {
     if (sparse) {
-        for (int k = lane_id(); k < n; k += 64) bits[list[k] >> 6] = 0ull;
+        for (int k = lane_id(); k < n; k += 64) bits[(list[k] & kIdMask) >> 6] = 0ull;
     } else {
         for (int j = lane_id(); j < W; j += 64) bits[j] = 0ull;
     }
@@ -441,12 +480,21 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     };
     if (has_wb && L > 0 && word_char(win.at(0))) wb_pass();
 
+    // first 64 always-active row entries of a byte (prefetched a step ahead)
+    auto load_alw = [&](int cc) -> uint32_t {
+        const int ab = g.sym_stats[cc].w;
+        const int ae = cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total;
+        return lane_id() < ae - ab ? g.alw_dst[ab + lane_id()] : 0u;
+    };
+    uint32_t pre = L > 0 ? load_alw(int(win.at(0))) : 0u;
+
     StepCtx ctx;
     ctx.len = L;
     ctx.last_m = M_INFA;
     int cached = -1;
     for (int64_t b = 0; b < L; ++b) {
         const int c = int(win.at(b));
+        const uint32_t pre_next = b + 1 < L ? load_alw(int(win.at(b + 1))) : 0u;
         ctx.c = c;
         ctx.n_act = n_act;
         ctx.sparse = sparse;
@@ -499,7 +547,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         bool fsp;
         int n_fut;
         if (!log_step) {
-            n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp);
+            n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp, pre);
         } else {
             // run every method on the same state and time each; keep the
             // iNFA result (all three are the same set)
@@ -512,7 +560,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 if (mm == M_AS && !sparse) continue;
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp);
+                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp, pre);
                 wave_sync();
                 if (!fsp) {  // a bitmap-only result pays for its list rebuild
                     n_fut = uniform(rebuild_list<CAP>(fut, W, fl));
@@ -566,6 +614,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             // win.at(b+1) may have moved the window forward: b is not read again
             if (boundary) wb_pass();
         }
+        pre = pre_next;
     }
     wave_sync();
     emit_accepts(g, a, sid, cur);

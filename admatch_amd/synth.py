@@ -377,7 +377,8 @@ def make_inputs(rules: Optional[Sequence[object]], n: int, lengths: np.ndarray, 
     rng = np.random.default_rng(seed)
     lengths = np.asarray(lengths, dtype=np.int64)
     total = int(lengths.sum())
-    corpus = _corpus(rng, total + 4096)
+    # strings are windows of a shared request corpus (at most 4 MB of text)
+    corpus = _corpus(rng, min(total, 4 << 20) + int(lengths.max()) + 4096)
     offsets = np.zeros(n + 1, dtype=np.int64)
     offsets[1:] = np.cumsum(lengths)
     starts = rng.integers(0, corpus.size - int(lengths.max()) - 1, size=n)

@@ -180,3 +180,49 @@ def test_merge_rule_ids(tmp_path):
     assert o.S == 5
     # natural sort puts rule9 first: its rule id is 1, rule10's is 2
     assert o.scan(b"b") == [1] and o.scan(b"a") == [2] and o.scan(b"xa") == []
+
+
+def _decode_as_tab(s, S):
+    tab = s.table(10).reshape(256, S)
+    multi = s.table(11)
+    rows = {}
+    for c in range(256):
+        for st in np.nonzero(tab[c])[0]:
+            e = int(tab[c, st])
+            cnt, pay = e >> 24, e & 0xFFFFFF
+            if cnt == 1:
+                rows[(c, int(st))] = [pay]
+            elif cnt == 255:
+                rows[(c, int(st))] = multi[pay + 1: pay + 1 + multi[pay]].tolist()
+            else:
+                rows[(c, int(st))] = multi[pay: pay + cnt].tolist()
+    return rows
+
+
+@pytest.mark.parametrize("name", ["crs_g6.nfa.gz", "synth_small.nfa.gz"])
+def test_device_as_table(gold_dir, name):
+    # device rows = AS rows plus the state itself for persistent states;
+    # always-active rows = merged rows of initial+persistent states
+    s, o = same_as_oracle(golden_nfa(name, gold_dir))
+    S = o.S
+    rows = _decode_as_tab(s, S)
+    off, tr = o.offsets(), o.transitions()
+    init, persis, _ = o.vectors()
+    pb = np.unpackbits(persis.view(np.uint8), bitorder="little")[:S].astype(bool)
+    ib = np.unpackbits(init.view(np.uint8), bitorder="little")[:S].astype(bool)
+    exp = {}
+    for c in range(256):
+        for a, b in tr[off[c]:off[c + 1]].tolist():
+            exp.setdefault((c, a), set()).add(b)
+        for p in np.nonzero(pb)[0]:
+            exp.setdefault((c, int(p)), set()).add(int(p))
+    assert set(rows) == set(exp)
+    for k, v in rows.items():
+        assert v == sorted(exp[k]), k
+    alw = np.nonzero(pb & ib)[0].tolist()
+    aoff, adst = s.table(12), s.table(13)
+    for c in (0, 97, 101, 255):
+        want = sorted(set().union(*[exp.get((c, a), set()) for a in alw])) if alw else []
+        got = adst[aoff[c]:aoff[c + 1]]
+        assert (got & 0x7FFFFFFF).tolist() == want
+        assert [bool(x >> 31) for x in got] == [bool(pb[d] and ib[d]) for d in want]
