@@ -76,14 +76,14 @@ def load(hash_: int):
         return json.load(f).get(str(hash_))
 
 
-def store(hash_: int, entry: Dict) -> None:
+def store(hash_: int, entry: Dict, path: str = CACHE) -> None:
     data = {}
-    if os.path.exists(CACHE):
-        with open(CACHE) as f:
+    if os.path.exists(path):
+        with open(path) as f:
             data = json.load(f)
     data[str(hash_)] = entry
-    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
-    with open(CACHE, "w") as f:
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
         json.dump(data, f, indent=1, sort_keys=True)
 
 

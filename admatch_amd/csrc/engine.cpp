@@ -309,7 +309,7 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     }
     ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
     int blocks = 0;
-    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, lds, &blocks));
+    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, mode, lds, &blocks));
     if (blocks <= 0) {
         err = "engine variant does not fit on a CU";
         return ADM_ERR_TOO_LARGE;
@@ -317,7 +317,7 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     int grid = blocks * cu_count_;
     const int need = (n + var_.wpb - 1) / var_.wpb;
     grid = std::max(1, std::min(grid, need));
-    ADM_HIP(launch_engine(var_.wpb, var_.cap, grid, lds, st, dev_, a));
+    ADM_HIP(launch_engine(var_.wpb, var_.cap, mode, grid, lds, st, dev_, a));
     return ADM_OK;
 }
 

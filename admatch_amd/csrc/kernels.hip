@@ -438,7 +438,7 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
     }
 }
 
-template <int CAP>
+template <int CAP, int MODE>
 __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
                             uint32_t* listA, uint32_t* listB, uint8_t* dcache, int* msteps) {
     const int W = g.W;
@@ -505,7 +505,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         ctx.Lft = st.z;
         int m;
         bool log_step = false;
-        switch (a.mode) {
+        switch (MODE) {
             case MODE_INFA: m = M_INFA; break;
             case MODE_TKO: m = M_TKO; break;
             case MODE_AS: m = sparse ? M_AS : M_INFA; break;
@@ -623,9 +623,14 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     wave_sync();
 }
 
-template <int WPB, int CAP>
+template <int WPB, int CAP, int MODE>
 __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a) {
     extern __shared__ uint64_t lds[];
+    // per-byte statistics shared by the workgroup's waves
+    __shared__ int4 s_stats[256];
+    for (int i = threadIdx.x; i < 256; i += 64 * WPB) s_stats[i] = g.sym_stats[i];
+    __syncthreads();
+    g.sym_stats = s_stats;
     const int wave = threadIdx.x >> 6;
     const size_t per = Lane<CAP>::words(g.W);
     uint64_t* base = lds + per * wave;
@@ -644,7 +649,7 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
         if (idx >= a.n) break;
         const int sid = a.order ? a.order[idx] : idx;
-        scan_string<CAP>(g, a, sid, bitsA, bitsB, listA, listB, dcache, msteps);
+        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, dcache, msteps);
     }
     if (a.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)
@@ -691,22 +696,41 @@ __global__ void bin_scatter_kernel(const int64_t* offsets, int32_t n, const int3
 
 // ------------------------------------------------------------ launchers
 namespace {
-template <int WPB, int CAP>
-hipError_t engine_launch_t(int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
+template <int WPB, int CAP, int MODE>
+hipError_t engine_launch_m(int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP, MODE>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((engine_kernel<WPB, CAP>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
+    hipLaunchKernelGGL((engine_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
     return hipGetLastError();
 }
-template <int WPB, int CAP>
-hipError_t engine_occ_t(size_t lds, int* blocks) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP>),
+template <int WPB, int CAP, int MODE>
+hipError_t engine_occ_m(size_t lds, int* blocks) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP, MODE>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, engine_kernel<WPB, CAP>, 64 * WPB, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, engine_kernel<WPB, CAP, MODE>, 64 * WPB, lds);
+}
+#define ADM_DISPATCH_MODE(FN, ...)                                         \
+    switch (mode) {                                                        \
+        case MODE_INFA: return FN<WPB, CAP, MODE_INFA>(__VA_ARGS__);       \
+        case MODE_TKO: return FN<WPB, CAP, MODE_TKO>(__VA_ARGS__);         \
+        case MODE_AS: return FN<WPB, CAP, MODE_AS>(__VA_ARGS__);           \
+        case MODE_ADMATCH: return FN<WPB, CAP, MODE_ADMATCH>(__VA_ARGS__); \
+        case MODE_HEUR: return FN<WPB, CAP, MODE_HEUR>(__VA_ARGS__);       \
+        case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED>(__VA_ARGS__); \
+        case MODE_LOG: return FN<WPB, CAP, MODE_LOG>(__VA_ARGS__);         \
+        default: return hipErrorInvalidValue;                              \
+    }
+template <int WPB, int CAP>
+hipError_t engine_launch_t(int mode, int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
+    ADM_DISPATCH_MODE(engine_launch_m, grid, lds, st, g, a)
+}
+template <int WPB, int CAP>
+hipError_t engine_occ_t(int mode, size_t lds, int* blocks) {
+    ADM_DISPATCH_MODE(engine_occ_m, lds, blocks)
 }
 template <int BS>
 hipError_t plain_launch_t(int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
@@ -735,13 +759,13 @@ hipError_t plain_launch_t(int n, size_t lds, hipStream_t st, const DevGraph& g, 
         default: return hipErrorInvalidValue;                          \
     }
 
-hipError_t launch_engine(int wpb, int cap, int grid, size_t lds, hipStream_t st, const DevGraph& g,
+hipError_t launch_engine(int wpb, int cap, int mode, int grid, size_t lds, hipStream_t st, const DevGraph& g,
                          const ScanArgs& a) {
-    ADM_DISPATCH_ENGINE(engine_launch_t, grid, lds, st, g, a)
+    ADM_DISPATCH_ENGINE(engine_launch_t, mode, grid, lds, st, g, a)
 }
 
-hipError_t engine_occupancy(int wpb, int cap, size_t lds, int* blocks) {
-    ADM_DISPATCH_ENGINE(engine_occ_t, lds, blocks)
+hipError_t engine_occupancy(int wpb, int cap, int mode, size_t lds, int* blocks) {
+    ADM_DISPATCH_ENGINE(engine_occ_t, mode, lds, blocks)
 }
 
 hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {

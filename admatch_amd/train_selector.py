@@ -41,6 +41,7 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tune", action="store_true", help="autotune the launch variant first")
+    ap.add_argument("--autotune-out", default=autotune.CACHE, help="autotune cache to update")
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -57,8 +58,8 @@ def main(argv=None):
     dtr = (E.DeviceArray(tr.inputs.buf), E.DeviceArray(tr.inputs.offsets))
     dev = (E.DeviceArray(ev.inputs.buf), E.DeviceArray(ev.inputs.offsets))
     if args.tune:
-        best = autotune.tune(s, *dev, ev.inputs.n, mode=E.MODE_HEURISTIC, log=log)
-        autotune.store(s.info()["hash"], best)
+        best = autotune.tune(s, *dev, ev.inputs.n, mode=E.MODE_AS, log=log)
+        autotune.store(s.info()["hash"], best, path=args.autotune_out)
     else:
         autotune.apply(s)
     log(f"variant {s.variant()}")
