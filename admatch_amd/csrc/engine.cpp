@@ -299,7 +299,7 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
         return ADM_OK;
     }
     const size_t lds = engine_lds_bytes(var_);
-    if (lds > max_lds_) {
+    if (lds + sizeof(int4) * 256 > max_lds_) {
         err = "rule set too large for the engine's LDS budget (lower the waves per workgroup)";
         return ADM_ERR_TOO_LARGE;
     }

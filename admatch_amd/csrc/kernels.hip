@@ -290,6 +290,11 @@ __device__ int multi_rows(const DevGraph& g, uint32_t start, int len, uint64_t* 
 template <int CAP>
 __device__ int as_step(const DevGraph& g, int c, int alw_b, int alw_n, uint32_t pre, const uint32_t* cl, int n_cur,
                        uint64_t* fut, uint32_t* fl) {
+    const uint32_t* tab = g.as_tab + size_t(c) * g.S;
+    // the first chunk's table gathers are in flight while the always-active
+    // rows are merged
+    const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
+    const uint32_t e0 = (sv0 & kTag) ? 0u : tab[sv0];
     int n = 0;
     for (int t0 = 0; t0 < alw_n; t0 += 64) {
         const int t = t0 + lane_id();
@@ -298,26 +303,37 @@ __device__ int as_step(const DevGraph& g, int c, int alw_b, int alw_n, uint32_t 
         const bool nw = ok && set_new(fut, v & kIdMask);
         n = list_append<CAP>(fl, n, nw, v);
     }
-    const uint32_t* tab = g.as_tab + size_t(c) * g.S;
     for (int k0 = 0; k0 < n_cur; k0 += 64) {
-        const int k = k0 + lane_id();
-        const uint32_t sv = k < n_cur ? cl[k] : kTag;
-        const uint32_t e = (sv & kTag) ? 0u : tab[sv];
-        const uint32_t cnt = e >> 24;
-        const bool single = cnt == 1;
-        const uint32_t d = e & kIdMask;
-        const bool nw = single && set_new(fut, d);
-        n = list_append<CAP>(fl, n, nw, d);
-        const bool multi = cnt >= 2;
-        if (__ballot(multi)) {
-            uint32_t start = d;
-            int len = multi ? int(cnt) : 0;
-            if (cnt == 255) {
-                len = int(g.as_multi[d]);
-                start = d + 1;
-            }
-            n = multi_rows<CAP>(g, start, len, fut, fl, n);
+        uint32_t e = e0;
+        if (k0 > 0) {
+            const int k = k0 + lane_id();
+            const uint32_t sv = k < n_cur ? cl[k] : kTag;
+            e = (sv & kTag) ? 0u : tab[sv];
         }
+        const uint32_t cnt = e >> 24;
+        const uint32_t d = e & kIdMask;
+        const bool nw = cnt == 1 && set_new(fut, d);
+        n = list_append<CAP>(fl, n, nw, d);
+        // short rows: each lane walks its own row, four loads in flight
+        const int len = (cnt >= 2 && cnt < 255) ? int(cnt) : 0;
+        if (__ballot(len > 0)) {
+            uint32_t x[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x[j] = j < len ? g.as_multi[d + j] : 0u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const bool nj = j < len && set_new(fut, x[j]);
+                n = list_append<CAP>(fl, n, nj, x[j]);
+            }
+            for (int j = 4; __ballot(j < len); ++j) {
+                const uint32_t y = j < len ? g.as_multi[d + j] : 0u;
+                const bool nj = j < len && set_new(fut, y);
+                n = list_append<CAP>(fl, n, nj, y);
+            }
+        }
+        // long rows (255 or more destinations): flattened over the wave
+        const bool lng = cnt == 255;
+        if (__ballot(lng)) n = multi_rows<CAP>(g, d + 1, lng ? int(g.as_multi[d]) : 0, fut, fl, n);
     }
     return n;
 }
@@ -696,12 +712,17 @@ __global__ void bin_scatter_kernel(const int64_t* offsets, int32_t n, const int3
 
 // ------------------------------------------------------------ launchers
 namespace {
+// dynamic LDS the engine may request: the CU's 160 KB minus the static
+// per-workgroup byte statistics
+constexpr int kEngineDynLds = 160 * 1024 - int(sizeof(int4)) * 256;
+
 template <int WPB, int CAP, int MODE>
 hipError_t engine_launch_m(int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP, MODE>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, kEngineDynLds);
+        (void)hipGetLastError();  // an unsupported attribute must not fail the launch check
         attr = true;
     }
     hipLaunchKernelGGL((engine_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
@@ -710,7 +731,8 @@ hipError_t engine_launch_m(int grid, size_t lds, hipStream_t st, const DevGraph&
 template <int WPB, int CAP, int MODE>
 hipError_t engine_occ_m(size_t lds, int* blocks) {
     (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP, MODE>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                              hipFuncAttributeMaxDynamicSharedMemorySize, kEngineDynLds);
+    (void)hipGetLastError();
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, engine_kernel<WPB, CAP, MODE>, 64 * WPB, lds);
 }
 #define ADM_DISPATCH_MODE(FN, ...)                                         \
@@ -737,7 +759,8 @@ hipError_t plain_launch_t(int n, size_t lds, hipStream_t st, const DevGraph& g, 
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&plain_infa_kernel<BS>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        (void)hipGetLastError();
         attr = true;
     }
     hipLaunchKernelGGL((plain_infa_kernel<BS>), dim3(n), dim3(BS), lds, st, g, a);
