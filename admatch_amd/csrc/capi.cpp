@@ -277,6 +277,8 @@ int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap) {
         case 11: return give(g.as_multi.data(), int64_t(g.as_multi.size()), 4);
         case 12: return give(g.alw_off.data(), int64_t(g.alw_off.size()), 4);
         case 13: return give(g.alw_dst.data(), int64_t(g.alw_dst.size()), 4);
+        case 14: return give(g.pair_off.data(), int64_t(g.pair_off.size()), 4);
+        case 15: return give(g.pair_dst.data(), int64_t(g.pair_dst.size()), 4);
         default: return fail(ITA_ERR_ARG, "bad table id");
     }
 }

@@ -23,6 +23,8 @@ struct DevGraph {
     const uint32_t* as_multi; // destinations of multi-entry rows
     const uint32_t* alw_dst;  // always-active rows per byte
     int32_t alw_total;        // alw_dst entries
+    const uint32_t* pair_off; // [65537] two-byte rows of first positions (null: none)
+    const uint32_t* pair_dst;
     const int4* sym_stats;    // [256]: |T_c|, k_c, |leftover_c|, start in alw_dst
     int32_t wb_begin, wb_count;  // word-boundary slice of list
 };

@@ -158,7 +158,7 @@ int Engine::load(const char* path, std::string& err) {
 }
 
 size_t Engine::engine_lds_bytes(const Variant& v) const {
-    return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap) + 32) * 8;
+    return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap)) * 8;
 }
 
 size_t Engine::plain_lds_bytes() const { return size_t(3) * g_.W * 8 + 258 * 4; }
@@ -246,6 +246,13 @@ int Engine::upload(std::string& err) {
     }
     ADM_PUT(g_.alw_dst, alw_dst)
     dev_.alw_total = g_.alw_off[256];
+    if (!g_.pair_off.empty()) {
+        ADM_PUT(g_.pair_off, pair_off)
+        ADM_PUT(g_.pair_dst, pair_dst)
+    } else {
+        dev_.pair_off = nullptr;
+        dev_.pair_dst = nullptr;
+    }
     std::vector<int4> st(256);
     for (int c = 0; c < 256; ++c)
         st[c] = make_int4(g_.sym_off[c + 1] - g_.sym_off[c], g_.tko_koff[c + 1] - g_.tko_koff[c],

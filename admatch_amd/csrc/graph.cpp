@@ -395,9 +395,45 @@ void build_as(Graph& g) {
         }
         std::sort(row.begin(), row.end());
         row.erase(std::unique(row.begin(), row.end()), row.end());
-        for (uint32_t d : row) g.alw_dst.push_back(always(d) ? (d | 0x80000000u) : d);
+        // bit 31: always active; bit 30: a first position (reached from an
+        // always-active state, not itself always active)
+        for (uint32_t d : row) g.alw_dst.push_back(always(d) ? (d | 0x80000000u) : (d | 0x40000000u));
     }
     g.alw_off[256] = int32_t(g.alw_dst.size());
+
+    // two-byte rows of the first positions
+    g.pair_off.assign(size_t(256) * 256 + 1, 0);
+    g.pair_dst.clear();
+    bool pair_ok = true;
+    std::vector<uint32_t> d1;
+    for (int c1 = 0; c1 < 256 && pair_ok; ++c1) {
+        d1.clear();
+        for (int32_t k = g.alw_off[c1]; k < g.alw_off[c1 + 1]; ++k)
+            if (!(g.alw_dst[k] & 0x80000000u)) d1.push_back(g.alw_dst[k] & 0x00ffffffu);
+        for (int c2 = 0; c2 < 256; ++c2) {
+            g.pair_off[size_t(c1) * 256 + c2] = uint32_t(g.pair_dst.size());
+            row.clear();
+            for (uint32_t s : d1) {
+                const uint32_t* b = g.as_dst.data() + g.as_off[size_t(s) * 256 + c2];
+                const uint32_t* e = g.as_dst.data() + g.as_off[size_t(s) * 256 + c2 + 1];
+                row.insert(row.end(), b, e);
+                if (persistent(s)) row.push_back(s);
+            }
+            std::sort(row.begin(), row.end());
+            row.erase(std::unique(row.begin(), row.end()), row.end());
+            if (g.pair_dst.size() + row.size() > kMaxPairEntries) {
+                pair_ok = false;
+                break;
+            }
+            for (uint32_t d : row) g.pair_dst.push_back(always(d) ? (d | 0x80000000u) : d);
+        }
+    }
+    if (!pair_ok) {
+        g.pair_off.clear();
+        g.pair_dst.clear();
+    } else {
+        g.pair_off[size_t(256) * 256] = uint32_t(g.pair_dst.size());
+    }
 }
 
 }  // namespace admatch

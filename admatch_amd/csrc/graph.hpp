@@ -22,6 +22,7 @@ namespace admatch {
 constexpr int kSymbols = 257;      // 256 byte values + word boundary (src/common.h:13)
 constexpr int kWordBoundary = 256;  // src/common.h:15
 constexpr int kMaxK = 1024;        // src/common.h:21
+constexpr size_t kMaxPairEntries = size_t(1) << 24;
 constexpr int32_t kMaxStates = 1 << 24;  // ids fit the 24-bit payload of the AS table
 
 struct Edge {
@@ -71,6 +72,12 @@ struct Graph {
     // alw_off[c+1]).
     std::vector<int32_t> alw_off;
     std::vector<uint32_t> alw_dst;
+    // Two-byte rows: the states the always-active rows reach on byte c1
+    // (the first positions, D1(c1)) are active at the next step whatever
+    // else happened, so their rows on the next byte c2 are merged ahead of
+    // time: pair_dst[pair_off[c1*256+c2] .. pair_off[c1*256+c2+1]).  Empty
+    // when the merged rows would exceed kMaxPairEntries.
+    std::vector<uint32_t> pair_off, pair_dst;
 
     // parse statistics
     int64_t n_persis_lines = 0, n_bad_lines = 0, n_raw_transitions = 0;

@@ -22,8 +22,11 @@ namespace admatch {
 
 // ------------------------------------------------------------------ common
 // List entries carry bit 31 for always-active states (their rows come from
-// the per-byte always-active list instead of the AS table).
+// the per-byte always-active list instead of the AS table) and bit 30 for
+// first positions added by the always-active rows of the previous byte
+// (their rows come from the two-byte rows).
 constexpr uint32_t kTag = 0x80000000u;
+constexpr uint32_t kFirst = 0x40000000u;
 constexpr uint32_t kIdMask = 0x00ffffffu;
 
 __device__ __forceinline__ bool set_new(uint64_t* bits, uint32_t d) {
@@ -287,28 +290,47 @@ __device__ int multi_rows(const DevGraph& g, uint32_t start, int len, uint64_t* 
 // rows of the active states on byte c.  Rows of persistent states contain
 // the state itself.  pre holds the first 64 always-active entries of c,
 // loaded one step ahead.
+// Prefetched slice of a precomputed row list: the first 64 entries, one per
+// lane, plus where the list lives.
+struct RowSlice {
+    uint32_t v;  // entry lane_id() (valid when lane_id() < n)
+    int b, n;    // start and length in the backing array
+};
+
 template <int CAP>
-__device__ int as_step(const DevGraph& g, int c, int alw_b, int alw_n, uint32_t pre, const uint32_t* cl, int n_cur,
-                       uint64_t* fut, uint32_t* fl) {
-    const uint32_t* tab = g.as_tab + size_t(c) * g.S;
-    // the first chunk's table gathers are in flight while the always-active
-    // rows are merged
-    const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
-    const uint32_t e0 = (sv0 & kTag) ? 0u : tab[sv0];
-    int n = 0;
-    for (int t0 = 0; t0 < alw_n; t0 += 64) {
+__device__ int merge_rows(const uint32_t* arr, const RowSlice& r, uint64_t* fut, uint32_t* fl, int n) {
+    for (int t0 = 0; t0 < r.n; t0 += 64) {
         const int t = t0 + lane_id();
-        const bool ok = t < alw_n;
-        const uint32_t v = t0 == 0 ? pre : (ok ? g.alw_dst[alw_b + t] : 0u);
+        const bool ok = t < r.n;
+        const uint32_t v = t0 == 0 ? r.v : (ok ? arr[r.b + t] : 0u);
         const bool nw = ok && set_new(fut, v & kIdMask);
         n = list_append<CAP>(fl, n, nw, v);
     }
+    return n;
+}
+
+// AS step (src/AS_kernel.cu:74-125 with sparse rows): fut = union of the
+// rows of the active states on byte c.  Rows of persistent states contain
+// the state itself.  alw: always-active rows of c; pair: two-byte rows of
+// (previous byte, c), with skip_first set when they are merged here, in
+// which case first positions (bit 30) need no table gather.
+template <int CAP>
+__device__ int as_step(const DevGraph& g, int c, const RowSlice& alw, const RowSlice& pair, bool skip_first,
+                       const uint32_t* cl, int n_cur, uint64_t* fut, uint32_t* fl) {
+    const uint32_t* tab = g.as_tab + size_t(c) * g.S;
+    const uint32_t skip = skip_first ? (kTag | kFirst) : kTag;
+    // the first chunk's table gathers are in flight while the precomputed
+    // rows are merged
+    const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
+    const uint32_t e0 = (sv0 & skip) ? 0u : tab[sv0 & kIdMask];
+    int n = merge_rows<CAP>(g.alw_dst, alw, fut, fl, 0);
+    if (skip_first) n = merge_rows<CAP>(g.pair_dst, pair, fut, fl, n);
     for (int k0 = 0; k0 < n_cur; k0 += 64) {
         uint32_t e = e0;
         if (k0 > 0) {
             const int k = k0 + lane_id();
             const uint32_t sv = k < n_cur ? cl[k] : kTag;
-            e = (sv & kTag) ? 0u : tab[sv];
+            e = (sv & skip) ? 0u : tab[sv & kIdMask];
         }
         const uint32_t cnt = e >> 24;
         const uint32_t d = e & kIdMask;
@@ -374,7 +396,7 @@ __device__ __forceinline__ void make_features(const DevGraph& g, const StepCtx& 
 // W1 is stored input-major [16][64], W2 as [64 in][64 out], so every
 // weight load is one coalesced 256-byte row.  Returns argmin of the three
 // predicted log step costs among the allowed methods.
-__device__ int mlp_choice(const float* w, const float* f, bool allow_as) {
+__device__ __attribute__((noinline)) int mlp_choice(const float* w, const float* f, bool allow_as) {
     const int j = lane_id();
     float h = w[kMlpB1 + j];
 #pragma unroll
@@ -404,15 +426,16 @@ __device__ int mlp_choice(const float* w, const float* f, bool allow_as) {
 // ------------------------------------------------------------ engine
 template <int CAP>
 struct Lane {
-    // u64 words per wave: two bitmaps, two lists, 256-byte decision cache
-    static constexpr size_t words(int W) { return size_t(2) * W + CAP + 32; }
+    // u64 words per wave: two bitmaps, two lists
+    static constexpr size_t words(int W) { return size_t(2) * W + CAP; }
 };
 
 // Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
 // Returns the fut count (list valid iff *fut_sparse).
 template <int CAP>
 __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, const uint32_t* cl, int n_cur,
-                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse, uint32_t pre) {
+                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse, const RowSlice& alw,
+                          const RowSlice& pair, bool use_pair) {
     int n = 0;
     const int W = g.W;
     if (m == M_TKO) {
@@ -423,9 +446,7 @@ __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, 
         return -1;
     }
     if (m == M_AS && sparse && g.as_tab != nullptr) {
-        const int ab = g.sym_stats[c].w;
-        const int ae = c < 255 ? g.sym_stats[c + 1].w : g.alw_total;
-        n = as_step<CAP>(g, c, ab, ae - ab, pre, cl, n_cur, fut, fl);
+        n = as_step<CAP>(g, c, alw, pair, use_pair, cl, n_cur, fut, fl);
         *fut_sparse = n <= CAP;
         return n;
     }
@@ -456,7 +477,7 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
 
 template <int CAP, int MODE>
 __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
-                            uint32_t* listA, uint32_t* listB, uint8_t* dcache, int* msteps) {
+                            uint32_t* listA, uint32_t* listB, uint32_t& dcache, int* msteps) {
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int64_t L = a.offsets[sid + 1] - beg;
@@ -496,13 +517,26 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     };
     if (has_wb && L > 0 && word_char(win.at(0))) wb_pass();
 
-    // first 64 always-active row entries of a byte (prefetched a step ahead)
-    auto load_alw = [&](int cc) -> uint32_t {
-        const int ab = g.sym_stats[cc].w;
-        const int ae = cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total;
-        return lane_id() < ae - ab ? g.alw_dst[ab + lane_id()] : 0u;
+    // precomputed rows of a byte (always-active) and of a byte pair
+    // (two-byte rows), their first 64 entries loaded a step ahead
+    auto load_alw = [&](int cc) -> RowSlice {
+        RowSlice r;
+        r.b = g.sym_stats[cc].w;
+        r.n = (cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total) - r.b;
+        r.v = lane_id() < r.n ? g.alw_dst[r.b + lane_id()] : 0u;
+        return r;
     };
-    uint32_t pre = L > 0 ? load_alw(int(win.at(0))) : 0u;
+    auto load_pair = [&](int c1, int c2) -> RowSlice {
+        RowSlice r;
+        const uint32_t* o = g.pair_off + (c1 * 256 + c2);
+        r.b = int(o[0]);
+        r.n = int(o[1]) - r.b;
+        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
+        return r;
+    };
+    const bool have_pair = g.pair_off != nullptr;
+    RowSlice alw = L > 0 ? load_alw(int(win.at(0))) : RowSlice{0u, 0, 0};
+    RowSlice pair{0u, 0, 0};
 
     StepCtx ctx;
     ctx.len = L;
@@ -510,27 +544,42 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     int cached = -1;
     for (int64_t b = 0; b < L; ++b) {
         const int c = int(win.at(b));
-        const uint32_t pre_next = b + 1 < L ? load_alw(int(win.at(b + 1))) : 0u;
+        // two-byte rows apply from the second byte on
+        const bool use_pair = have_pair && b > 0;
+        RowSlice alw_next{0u, 0, 0}, pair_next{0u, 0, 0};
+        if (b + 1 < L) {
+            const int cn = int(win.at(b + 1));
+            alw_next = load_alw(cn);
+            if (have_pair) pair_next = load_pair(c, cn);
+        }
         ctx.c = c;
         ctx.n_act = n_act;
         ctx.sparse = sparse;
         ctx.pos = b;
-        const int4 st = g.sym_stats[c];
-        ctx.T = st.x;
-        ctx.K = st.y;
-        ctx.Lft = st.z;
+        // per-byte statistics, read only by the selectors that use them
+        auto load_stats = [&]() {
+            const int4 st = g.sym_stats[c];
+            ctx.T = st.x;
+            ctx.K = st.y;
+            ctx.Lft = st.z;
+        };
         int m;
         bool log_step = false;
         switch (MODE) {
             case MODE_INFA: m = M_INFA; break;
             case MODE_TKO: m = M_TKO; break;
             case MODE_AS: m = sparse ? M_AS : M_INFA; break;
-            case MODE_ADMATCH: m = (sparse && ctx.K > n_act) ? M_AS : M_TKO; break;
+            case MODE_ADMATCH:
+                load_stats();
+                m = (sparse && ctx.K > n_act) ? M_AS : M_TKO;
+                break;
             case MODE_LEARNED:
                 if (a.mlp == nullptr) {
+                    load_stats();
                     m = heuristic_choice(g, ctx);
                 } else if (a.mlp_every > 0) {
                     // re-evaluate every mlp_every steps
+                    load_stats();
                     if (cached < 0 || (b % a.mlp_every) == 0) {
                         float f[kFeatures];
                         make_features(g, ctx, f);
@@ -539,31 +588,38 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                     m = cached;
                     if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
                 } else {
-                    // per-wave decision cache keyed by the byte, tagged with
-                    // the active-count bucket and list validity
+                    // per-wave decision cache in one VGPR: byte c's entry is
+                    // byte (c & 3) of lane c >> 2, tagged with the active-count
+                    // bucket and list validity
                     const int tag = ((31 - __clz(n_act + 1)) << 1) | int(sparse);
-                    const int e = dcache[c];
+                    const int sh = (c & 3) * 8;
+                    const int e = int((uint32_t(__builtin_amdgcn_readlane(int(dcache), c >> 2)) >> sh) & 0xffu);
                     if ((e >> 2) == tag) {
                         m = e & 3;
                     } else {
+                        load_stats();
                         float f[kFeatures];
                         make_features(g, ctx, f);
                         m = mlp_choice(a.mlp, f, sparse);
-                        if (lane_id() == 0) dcache[c] = uint8_t((tag << 2) | m);
-                        wave_sync();
+                        if (lane_id() == (c >> 2))
+                            dcache = (dcache & ~(0xffu << sh)) | (uint32_t((tag << 2) | m) << sh);
                     }
                 }
                 break;
             case MODE_LOG:
+                load_stats();
                 m = heuristic_choice(g, ctx);
                 log_step = ((uint64_t(sid) * 2654435761ull + uint64_t(b)) % uint64_t(a.log_stride)) == 0;
                 break;
-            default: m = heuristic_choice(g, ctx); break;
+            default:
+                load_stats();
+                m = heuristic_choice(g, ctx);
+                break;
         }
         bool fsp;
         int n_fut;
         if (!log_step) {
-            n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp, pre);
+            n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp, alw, pair, use_pair);
         } else {
             // run every method on the same state and time each; keep the
             // iNFA result (all three are the same set)
@@ -576,7 +632,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 if (mm == M_AS && !sparse) continue;
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp, pre);
+                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp, alw, pair, use_pair);
                 wave_sync();
                 if (!fsp) {  // a bitmap-only result pays for its list rebuild
                     n_fut = uniform(rebuild_list<CAP>(fut, W, fl));
@@ -630,7 +686,8 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             // win.at(b+1) may have moved the window forward: b is not read again
             if (boundary) wb_pass();
         }
-        pre = pre_next;
+        alw = alw_next;
+        pair = pair_next;
     }
     wave_sync();
     emit_accepts(g, a, sid, cur);
@@ -654,9 +711,8 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
     uint64_t* bitsB = base + g.W;
     uint32_t* listA = reinterpret_cast<uint32_t*>(base + 2 * g.W);
     uint32_t* listB = listA + CAP;
-    uint8_t* dcache = reinterpret_cast<uint8_t*>(base + 2 * g.W + CAP);
+    uint32_t dcache = ~0u;  // learned selector: no cached decisions
     for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
-    for (int j = lane_id(); j < 32; j += 64) base[2 * g.W + CAP + j] = ~0ull;  // no cached decisions
     wave_sync();
     int msteps[3] = {0, 0, 0};  // steps per method, this wave
     for (;;) {

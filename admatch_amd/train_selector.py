@@ -42,6 +42,8 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--tune", action="store_true", help="autotune the launch variant first")
     ap.add_argument("--autotune-out", default=autotune.CACHE, help="autotune cache to update")
+    ap.add_argument("--extra-medium", type=int, default=4096,
+                    help="also log this many strings on the medium rule set (0: large only)")
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -66,6 +68,17 @@ def main(argv=None):
 
     t0 = time.time()
     rec = selector.collect(s, *dtr, tr.inputs.n, cap=1 << 21, stride=args.stride)
+    if args.extra_medium > 0:
+        # a second rule set (1,000 rules, W = 356 words) so that the network
+        # sees more than one state-vector size
+        mw = synth.make_workload("medium", n=args.extra_medium, seed_offset=1000)
+        mpath = os.path.join(d, "medium.nfa")
+        mw.ruleset.write(mpath)
+        ms = E.Scratch(E.AD_MATCHER, mpath).alloc_scratch()
+        autotune.apply(ms)
+        dm = (E.DeviceArray(mw.inputs.buf), E.DeviceArray(mw.inputs.offsets))
+        rec = np.concatenate([rec, selector.collect(ms, *dm, mw.inputs.n, cap=1 << 20, stride=args.stride // 2)])
+        ms.close()
     ds = selector.from_log(rec)
     log(f"training log: {ds.n} steps in {time.time() - t0:.1f} s; best method counts "
         f"{np.bincount(ds.best(), minlength=3).tolist()}")

@@ -3,9 +3,10 @@
 A "step" is one BatchedScan of the whole synthetic batch, inputs already
 resident in HBM: the engine kernel (plus its two small length-binning
 kernels) replayed from a hipGraph, then the accepted states copied back and
-mapped to sorted rule ids on the host.  The default workload is the medium
-configuration: 1,000 seed-generated rules (~22.8k NFA states) against
-16,384 HTTP-like strings with lognormal lengths (median 600 B, max 8 KB).
+mapped to sorted rule ids on the host.  The default workload is the large
+configuration the learned selector is specified on: 3,000 seed-generated
+rules (83,408 NFA states, 30% with '.*' gaps) against 65,536 HTTP-like
+strings with lognormal lengths (median 1 KB, max 16 KB, 92 MB in total).
 
 Printed: ONE JSON line with input GB/s of the headline selector, the same
 for the plain single-kernel iNFA path and the heuristic selectors, per
@@ -82,7 +83,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="medium", choices=["small", "medium", "large"])
+    ap.add_argument("--config", default="large", choices=["small", "medium", "large"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="plain,admatch,heuristic,as,learned")

@@ -137,7 +137,8 @@ int ita_info(ita_handle* h, int64_t* out11);
  * 6 TKO leftover (2*n uint32), 7 AS offsets (S*256+1 uint32), 8 AS dst
  * (uint32), 9 init/persistent/accept vectors (3*W u64), 10 device AS
  * table (256*S uint32), 11 its multi-row store (uint32), 12 always-active
- * row offsets (257 int32), 13 always-active rows (uint32).  Returns the
+ * row offsets (257 int32), 13 always-active rows (uint32), 14 two-byte
+ * row offsets (65537 uint32), 15 two-byte rows (uint32).  Returns the
  * element count; copies min(count, cap) elements when out != NULL. */
 int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap);
 

@@ -224,5 +224,14 @@ def test_device_as_table(gold_dir, name):
     for c in (0, 97, 101, 255):
         want = sorted(set().union(*[exp.get((c, a), set()) for a in alw])) if alw else []
         got = adst[aoff[c]:aoff[c + 1]]
-        assert (got & 0x7FFFFFFF).tolist() == want
-        assert [bool(x >> 31) for x in got] == [bool(pb[d] and ib[d]) for d in want]
+        assert (got & 0xFFFFFF).tolist() == want
+        always = [bool(pb[d] and ib[d]) for d in want]
+        assert [bool(x >> 31) for x in got] == always
+        assert [bool((x >> 30) & 1) for x in got] == [not a for a in always]
+    # two-byte rows: rows on c2 of the first positions reached on c1
+    poff, pdst = s.table(14), s.table(15)
+    for c1, c2 in ((97, 98), (101, 120), (47, 97), (0, 0)):
+        first = [int(x & 0xFFFFFF) for x in adst[aoff[c1]:aoff[c1 + 1]] if not (x >> 31)]
+        want = sorted(set().union(*[exp.get((c2, a), set()) for a in first])) if first else []
+        got = pdst[poff[c1 * 256 + c2]:poff[c1 * 256 + c2 + 1]]
+        assert (got & 0xFFFFFF).tolist() == want, (c1, c2)
