@@ -70,7 +70,7 @@ __device__ __forceinline__ int list_append(uint32_t* list, int n, bool pred, uin
     const uint64_t m = __ballot(pred);
     if (pred) {
         const int idx = n + int(mbcnt(m));
-        if (idx < CAP) list[idx] = v;
+        if (unsigned(idx) < unsigned(CAP)) list[idx] = v;
     }
     return n + __popcll(m);
 }
@@ -286,10 +286,6 @@ __device__ int multi_rows(const DevGraph& g, uint32_t start, int len, uint64_t* 
     return n;
 }
 
-// AS step (src/AS_kernel.cu:74-125 with sparse rows): fut = union of the
-// rows of the active states on byte c.  Rows of persistent states contain
-// the state itself.  pre holds the first 64 always-active entries of c,
-// loaded one step ahead.
 // Prefetched slice of a precomputed row list: the first 64 entries, one per
 // lane, plus where the list lives.
 struct RowSlice {
@@ -297,40 +293,38 @@ struct RowSlice {
     int b, n;    // start and length in the backing array
 };
 
-template <int CAP>
-__device__ int merge_rows(const uint32_t* arr, const RowSlice& r, uint64_t* fut, uint32_t* fl, int n) {
-    for (int t0 = 0; t0 < r.n; t0 += 64) {
-        const int t = t0 + lane_id();
-        const bool ok = t < r.n;
-        const uint32_t v = t0 == 0 ? r.v : (ok ? arr[r.b + t] : 0u);
-        const bool nw = ok && set_new(fut, v & kIdMask);
-        n = list_append<CAP>(fl, n, nw, v);
-    }
-    return n;
-}
-
 // AS step (src/AS_kernel.cu:74-125 with sparse rows): fut = union of the
 // rows of the active states on byte c.  Rows of persistent states contain
-// the state itself.  alw: always-active rows of c; pair: two-byte rows of
-// (previous byte, c), with skip_first set when they are merged here, in
-// which case first positions (bit 30) need no table gather.
+// the state itself.  The always-active states and the first positions they
+// reached on the previous byte are held implicitly (the caller records the
+// byte): their rows on c are the precomputed two-byte rows (pair, merged
+// here when use_pair) and the always-active rows of c, which become the
+// next step's implicit set.  Always-active states (bit 31) are therefore
+// neither gathered nor stored.
 template <int CAP>
-__device__ int as_step(const DevGraph& g, int c, const RowSlice& alw, const RowSlice& pair, bool skip_first,
-                       const uint32_t* cl, int n_cur, uint64_t* fut, uint32_t* fl) {
+__device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_pair, const uint32_t* cl, int n_cur,
+                       uint64_t* fut, uint32_t* fl) {
     const uint32_t* tab = g.as_tab + size_t(c) * g.S;
-    const uint32_t skip = skip_first ? (kTag | kFirst) : kTag;
-    // the first chunk's table gathers are in flight while the precomputed
-    // rows are merged
+    // the first chunk's table gathers are in flight while the two-byte rows
+    // are merged
     const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
-    const uint32_t e0 = (sv0 & skip) ? 0u : tab[sv0 & kIdMask];
-    int n = merge_rows<CAP>(g.alw_dst, alw, fut, fl, 0);
-    if (skip_first) n = merge_rows<CAP>(g.pair_dst, pair, fut, fl, n);
+    const uint32_t e0 = (sv0 & kTag) ? 0u : tab[sv0 & kIdMask];
+    int n = 0;
+    if (use_pair) {
+        for (int t0 = 0; t0 < pair.n; t0 += 64) {
+            const int t = t0 + lane_id();
+            const uint32_t v = t0 == 0 ? pair.v : (t < pair.n ? g.pair_dst[pair.b + t] : kTag);
+            const bool ok = t < pair.n && !(v & kTag);
+            const bool nw = ok && set_new(fut, v & kIdMask);
+            n = list_append<CAP>(fl, n, nw, v & kIdMask);
+        }
+    }
     for (int k0 = 0; k0 < n_cur; k0 += 64) {
         uint32_t e = e0;
         if (k0 > 0) {
             const int k = k0 + lane_id();
             const uint32_t sv = k < n_cur ? cl[k] : kTag;
-            e = (sv & skip) ? 0u : tab[sv & kIdMask];
+            e = (sv & kTag) ? 0u : tab[sv & kIdMask];
         }
         const uint32_t cnt = e >> 24;
         const uint32_t d = e & kIdMask;
@@ -362,8 +356,8 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& alw, const RowS
 
 // ------------------------------------------------------------ selector
 struct StepCtx {
-    int c, n_act, sparse, T, K, Lft, last_m;
-    int64_t pos, len;
+    int c, c_prev, n_act, sparse, T, K, Lft, last_m;
+    int pos, len;
 };
 
 __device__ __forceinline__ int heuristic_choice(const DevGraph& g, const StepCtx& s) {
@@ -434,8 +428,7 @@ struct Lane {
 // Returns the fut count (list valid iff *fut_sparse).
 template <int CAP>
 __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, const uint32_t* cl, int n_cur,
-                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse, const RowSlice& alw,
-                          const RowSlice& pair, bool use_pair) {
+                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse) {
     int n = 0;
     const int W = g.W;
     if (m == M_TKO) {
@@ -444,11 +437,6 @@ __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, 
         scatter<CAP, false>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, fl, 0);
         *fut_sparse = false;
         return -1;
-    }
-    if (m == M_AS && sparse && g.as_tab != nullptr) {
-        n = as_step<CAP>(g, c, alw, pair, use_pair, cl, n_cur, fut, fl);
-        *fut_sparse = n <= CAP;
-        return n;
     }
     if (sparse) {
         n = persist_sparse<CAP>(g, cl, n_cur, fut, fl);
@@ -480,7 +468,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                             uint32_t* listA, uint32_t* listB, uint32_t& dcache, int* msteps) {
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
-    const int64_t L = a.offsets[sid + 1] - beg;
+    const int L = int(a.offsets[sid + 1] - beg);
     uint64_t* cur = bitsA;
     uint64_t* fut = bitsB;
     uint32_t* cl = listA;
@@ -493,10 +481,38 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     ByteWindow win{a.text + beg, L, 0, 0u};
     if (L > 0) win.load(0);
     const bool has_wb = g.wb_count > 0;
+    const bool have_pair = g.pair_off != nullptr;
+    // Byte whose always-active row is active but held implicitly (not in the
+    // bitmap or list), or -1.  Set by AS steps; everything that needs the full
+    // set (other methods, the word-boundary pass, acceptance) first calls
+    // materialize().
+    int imp = -1;
+    auto alw_range = [&](int cc, int& b0, int& n0) {
+        b0 = g.sym_stats[cc].w;
+        n0 = (cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total) - b0;
+    };
+    auto materialize = [&]() {
+        if (imp < 0) return;
+        int ab, an;
+        alw_range(imp, ab, an);
+        for (int t0 = 0; t0 < an; t0 += 64) {
+            const int t = t0 + lane_id();
+            const bool ok = t < an;
+            const uint32_t v = ok ? g.alw_dst[ab + t] : 0u;
+            const bool nw = ok && set_new(cur, v & kIdMask);
+            // always-active states keep bit 31: their rows are the always-active rows
+            n_act = list_append<CAP>(cl, n_act, nw, v & (kIdMask | kTag));
+        }
+        n_act = uniform(n_act);
+        if (n_act > CAP) sparse = false;
+        imp = -1;
+        wave_sync();
+    };
 
     auto wb_pass = [&]() {
         // in place, list order, one lane (thread_count == 1 semantics of
         // src/iNFA_kernel.cu:145-164)
+        materialize();
         int n = n_act;
         if (lane_id() == 0) {
             for (int t = 0; t < g.wb_count; ++t) {
@@ -517,43 +533,35 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     };
     if (has_wb && L > 0 && word_char(win.at(0))) wb_pass();
 
-    // precomputed rows of a byte (always-active) and of a byte pair
-    // (two-byte rows), their first 64 entries loaded a step ahead
-    auto load_alw = [&](int cc) -> RowSlice {
-        RowSlice r;
-        r.b = g.sym_stats[cc].w;
-        r.n = (cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total) - r.b;
-        r.v = lane_id() < r.n ? g.alw_dst[r.b + lane_id()] : 0u;
-        return r;
-    };
+    // two-byte rows of (byte, next byte), first 64 entries loaded a step ahead
     auto load_pair = [&](int c1, int c2) -> RowSlice {
         RowSlice r;
         const uint32_t* o = g.pair_off + (c1 * 256 + c2);
         r.b = int(o[0]);
         r.n = int(o[1]) - r.b;
-        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
+        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : kTag;
         return r;
     };
-    const bool have_pair = g.pair_off != nullptr;
-    RowSlice alw = L > 0 ? load_alw(int(win.at(0))) : RowSlice{0u, 0, 0};
-    RowSlice pair{0u, 0, 0};
+    RowSlice pair{kTag, 0, 0};
+    const bool can_as = g.as_tab != nullptr;
 
     StepCtx ctx;
     ctx.len = L;
     ctx.last_m = M_INFA;
+    ctx.c_prev = -1;
     int cached = -1;
-    for (int64_t b = 0; b < L; ++b) {
+    for (int b = 0; b < L; ++b) {
         const int c = int(win.at(b));
-        // two-byte rows apply from the second byte on
-        const bool use_pair = have_pair && b > 0;
-        RowSlice alw_next{0u, 0, 0}, pair_next{0u, 0, 0};
-        if (b + 1 < L) {
-            const int cn = int(win.at(b + 1));
-            alw_next = load_alw(cn);
-            if (have_pair) pair_next = load_pair(c, cn);
+        RowSlice pair_next{kTag, 0, 0};
+        if (have_pair && b + 1 < L) pair_next = load_pair(c, int(win.at(b + 1)));
+        int n_tot = n_act;
+        if (imp >= 0) {
+            int ab, an;
+            alw_range(imp, ab, an);
+            n_tot += an;
         }
         ctx.c = c;
-        ctx.n_act = n_act;
+        ctx.n_act = n_tot;
         ctx.sparse = sparse;
         ctx.pos = b;
         // per-byte statistics, read only by the selectors that use them
@@ -571,7 +579,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             case MODE_AS: m = sparse ? M_AS : M_INFA; break;
             case MODE_ADMATCH:
                 load_stats();
-                m = (sparse && ctx.K > n_act) ? M_AS : M_TKO;
+                m = (sparse && ctx.K > n_tot) ? M_AS : M_TKO;
                 break;
             case MODE_LEARNED:
                 if (a.mlp == nullptr) {
@@ -591,7 +599,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                     // per-wave decision cache in one VGPR: byte c's entry is
                     // byte (c & 3) of lane c >> 2, tagged with the active-count
                     // bucket and list validity
-                    const int tag = ((31 - __clz(n_act + 1)) << 1) | int(sparse);
+                    const int tag = ((31 - __clz(n_tot + 1)) << 1) | int(sparse);
                     const int sh = (c & 3) * 8;
                     const int e = int((uint32_t(__builtin_amdgcn_readlane(int(dcache), c >> 2)) >> sh) & 0xffu);
                     if ((e >> 2) == tag) {
@@ -616,23 +624,39 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 m = heuristic_choice(g, ctx);
                 break;
         }
+        if (m == M_AS && !(sparse && can_as)) m = M_INFA;
         bool fsp;
         int n_fut;
+        int next_imp = -1;
         if (!log_step) {
-            n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp, alw, pair, use_pair);
+            if (m == M_AS) {
+                n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, fut, fl);
+                fsp = n_fut <= CAP;
+                next_imp = c;
+            } else {
+                materialize();
+                n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp);
+            }
         } else {
-            // run every method on the same state and time each; keep the
-            // iNFA result (all three are the same set)
+            // time every method on the same state: AS on the implicit form,
+            // then TKO and iNFA on the materialised set; keep the iNFA result
             float tm[3] = {-1.f, -1.f, -1.f};
-            const int order[3] = {M_TKO, M_AS, M_INFA};
-            n_fut = 0;
-            fsp = false;
-            for (int oi = 0; oi < 3; ++oi) {
-                const int mm = order[oi];
-                if (mm == M_AS && !sparse) continue;
+            if (sparse && can_as) {
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp, alw, pair, use_pair);
+                n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, fut, fl);
+                fsp = n_fut <= CAP;
+                wave_sync();
+                __builtin_amdgcn_s_waitcnt(0);
+                tm[M_AS] = float(clock64() - t0);
+                clear_bits<CAP>(fut, fl, n_fut, fsp, W);
+                wave_sync();
+            }
+            materialize();
+            for (int mm = M_TKO; mm >= M_INFA; --mm) {
+                __builtin_amdgcn_s_waitcnt(0);
+                const uint64_t t0 = clock64();
+                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp);
                 wave_sync();
                 if (!fsp) {  // a bitmap-only result pays for its list rebuild
                     n_fut = uniform(rebuild_list<CAP>(fut, W, fl));
@@ -640,9 +664,8 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 }
                 wave_sync();
                 __builtin_amdgcn_s_waitcnt(0);
-                const uint64_t t1 = clock64();
-                tm[mm] = float(t1 - t0);
-                if (oi < 2) {
+                tm[mm] = float(clock64() - t0);
+                if (mm == M_TKO) {
                     clear_bits<CAP>(fut, fl, n_fut, fsp, W);
                     wave_sync();
                 }
@@ -671,10 +694,17 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         cl = fl;
         fl = tl;
         wave_sync();
+        imp = next_imp;
+        ctx.c_prev = c;
         if (fsp) {
             n_act = n_fut;
             sparse = true;
         } else {
+            // bitmap only: fold the implicit set in, then count (and list
+            // the states if they fit)
+            n_act = 0;
+            sparse = false;
+            materialize();
             n_act = uniform(rebuild_list<CAP>(cur, W, cl));
             sparse = n_act <= CAP;
         }
@@ -686,9 +716,9 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             // win.at(b+1) may have moved the window forward: b is not read again
             if (boundary) wb_pass();
         }
-        alw = alw_next;
         pair = pair_next;
     }
+    materialize();
     wave_sync();
     emit_accepts(g, a, sid, cur);
     wave_sync();
