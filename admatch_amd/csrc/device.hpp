@@ -25,6 +25,9 @@ struct DevGraph {
     int32_t alw_total;        // alw_dst entries
     const uint32_t* pair_off; // [65537] two-byte rows of first positions (null: none)
     const uint32_t* pair_dst;
+    const uint32_t* alw_states;  // always-active states (initial and persistent)
+    int32_t n_alw_states;
+    const uint64_t* init_explicit;  // [W] initial states that are not always active
     const int4* sym_stats;    // [256]: |T_c|, k_c, |leftover_c|, start in alw_dst
     int32_t wb_begin, wb_count;  // word-boundary slice of list
 };

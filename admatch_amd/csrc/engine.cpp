@@ -246,6 +246,9 @@ int Engine::upload(std::string& err) {
     }
     ADM_PUT(g_.alw_dst, alw_dst)
     dev_.alw_total = g_.alw_off[256];
+    ADM_PUT(g_.alw_states, alw_states)
+    dev_.n_alw_states = int32_t(g_.alw_states.size());
+    ADM_PUT(g_.init_explicit, init_explicit)
     if (!g_.pair_off.empty()) {
         ADM_PUT(g_.pair_off, pair_off)
         ADM_PUT(g_.pair_dst, pair_dst)

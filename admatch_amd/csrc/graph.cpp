@@ -354,6 +354,8 @@ void build_as(Graph& g) {
             row.assign(b, e);
             if (p && !std::binary_search(row.begin(), row.end(), uint32_t(s)))
                 row.insert(std::lower_bound(row.begin(), row.end(), uint32_t(s)), uint32_t(s));
+            // always-active destinations are held implicitly by the AS step
+            row.erase(std::remove_if(row.begin(), row.end(), always), row.end());
             uint32_t& ent = g.as_tab[size_t(c) * g.S + s];
             if (row.empty()) continue;
             if (row.size() == 1) {
@@ -382,8 +384,12 @@ void build_as(Graph& g) {
     g.alw_off.assign(257, 0);
     g.alw_dst.clear();
     std::vector<uint32_t> alw;
-    for (int32_t w = 0; w < g.W; ++w)
+    g.init_explicit.assign(size_t(g.W), 0);
+    for (int32_t w = 0; w < g.W; ++w) {
+        g.init_explicit[w] = g.init[w] & ~g.persis[w];
         for (uint64_t v = g.persis[w] & g.init[w]; v; v &= v - 1) alw.push_back(uint32_t(w * 64 + __builtin_ctzll(v)));
+    }
+    g.alw_states = alw;
     for (int c = 0; c < 256; ++c) {
         g.alw_off[c] = int32_t(g.alw_dst.size());
         row.clear();
@@ -425,7 +431,8 @@ void build_as(Graph& g) {
                 pair_ok = false;
                 break;
             }
-            for (uint32_t d : row) g.pair_dst.push_back(always(d) ? (d | 0x80000000u) : d);
+            for (uint32_t d : row)
+                if (!always(d)) g.pair_dst.push_back(d);
         }
     }
     if (!pair_ok) {

@@ -63,7 +63,8 @@ struct Graph {
     // long row whose count is as_multi[idx]), then either the single
     // destination (count 1) or the row's index into as_multi.  Rows of
     // persistent states include the state itself, so the AS step needs no
-    // separate persistent mask.
+    // separate persistent mask.  Always-active destinations are left out:
+    // the AS step holds those states implicitly.
     std::vector<uint32_t> as_tab, as_multi;
     bool as_ok = false;  // false when the rows do not fit the 24-bit index
     // Always-active states (initial and persistent: active at every step).
@@ -72,6 +73,8 @@ struct Graph {
     // alw_off[c+1]).
     std::vector<int32_t> alw_off;
     std::vector<uint32_t> alw_dst;
+    std::vector<uint32_t> alw_states;    // the always-active states
+    std::vector<uint64_t> init_explicit; // initial states that are not always active
     // Two-byte rows: the states the always-active rows reach on byte c1
     // (the first positions, D1(c1)) are active at the next step whatever
     // else happened, so their rows on the next byte c2 are merged ahead of

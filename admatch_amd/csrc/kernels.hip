@@ -313,10 +313,10 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
     if (use_pair) {
         for (int t0 = 0; t0 < pair.n; t0 += 64) {
             const int t = t0 + lane_id();
-            const uint32_t v = t0 == 0 ? pair.v : (t < pair.n ? g.pair_dst[pair.b + t] : kTag);
-            const bool ok = t < pair.n && !(v & kTag);
-            const bool nw = ok && set_new(fut, v & kIdMask);
-            n = list_append<CAP>(fl, n, nw, v & kIdMask);
+            const bool ok = t < pair.n;
+            const uint32_t v = t0 == 0 ? pair.v : (ok ? g.pair_dst[pair.b + t] : 0u);
+            const bool nw = ok && set_new(fut, v);
+            n = list_append<CAP>(fl, n, nw, v);
         }
     }
     for (int k0 = 0; k0 < n_cur; k0 += 64) {
@@ -473,8 +473,10 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     uint64_t* fut = bitsB;
     uint32_t* cl = listA;
     uint32_t* fl = listB;
-    // invariant: both bitmaps are zero here
-    for (int j = lane_id(); j < W; j += 64) cur[j] = g.init[j];
+    // invariant: both bitmaps are zero here.  The always-active states
+    // (initial and persistent) start implicit; the other initial states are
+    // listed.
+    for (int j = lane_id(); j < W; j += 64) cur[j] = g.init_explicit[j];
     wave_sync();
     int n_act = uniform(rebuild_list<CAP>(cur, W, cl));
     bool sparse = n_act <= CAP;
@@ -482,23 +484,29 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     if (L > 0) win.load(0);
     const bool has_wb = g.wb_count > 0;
     const bool have_pair = g.pair_off != nullptr;
-    // Byte whose always-active row is active but held implicitly (not in the
-    // bitmap or list), or -1.  Set by AS steps; everything that needs the full
-    // set (other methods, the word-boundary pass, acceptance) first calls
-    // materialize().
-    int imp = -1;
+    // States active but held implicitly (in neither the bitmap nor the list):
+    // imp >= 0: the always-active row of byte imp (the always-active states
+    // and the first positions they reached on that byte); imp == -2: the
+    // always-active states alone; imp == -1: none.  Set by AS steps;
+    // everything that needs the full set (other methods, the word-boundary
+    // pass, acceptance) first calls materialize().
+    int imp = -2;
     auto alw_range = [&](int cc, int& b0, int& n0) {
         b0 = g.sym_stats[cc].w;
         n0 = (cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total) - b0;
     };
     auto materialize = [&]() {
-        if (imp < 0) return;
-        int ab, an;
-        alw_range(imp, ab, an);
+        if (imp == -1) return;
+        int ab = 0, an = g.n_alw_states;
+        const uint32_t* src = g.alw_states;
+        if (imp >= 0) {
+            alw_range(imp, ab, an);
+            src = g.alw_dst;
+        }
         for (int t0 = 0; t0 < an; t0 += 64) {
             const int t = t0 + lane_id();
             const bool ok = t < an;
-            const uint32_t v = ok ? g.alw_dst[ab + t] : 0u;
+            const uint32_t v = ok ? (imp >= 0 ? src[ab + t] : (src[t] | kTag)) : 0u;
             const bool nw = ok && set_new(cur, v & kIdMask);
             // always-active states keep bit 31: their rows are the always-active rows
             n_act = list_append<CAP>(cl, n_act, nw, v & (kIdMask | kTag));
@@ -539,10 +547,10 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         const uint32_t* o = g.pair_off + (c1 * 256 + c2);
         r.b = int(o[0]);
         r.n = int(o[1]) - r.b;
-        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : kTag;
+        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
         return r;
     };
-    RowSlice pair{kTag, 0, 0};
+    RowSlice pair{0u, 0, 0};
     const bool can_as = g.as_tab != nullptr;
 
     StepCtx ctx;
@@ -552,13 +560,15 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     int cached = -1;
     for (int b = 0; b < L; ++b) {
         const int c = int(win.at(b));
-        RowSlice pair_next{kTag, 0, 0};
+        RowSlice pair_next{0u, 0, 0};
         if (have_pair && b + 1 < L) pair_next = load_pair(c, int(win.at(b + 1)));
         int n_tot = n_act;
         if (imp >= 0) {
             int ab, an;
             alw_range(imp, ab, an);
             n_tot += an;
+        } else if (imp == -2) {
+            n_tot += g.n_alw_states;
         }
         ctx.c = c;
         ctx.n_act = n_tot;
