@@ -158,7 +158,7 @@ int Engine::load(const char* path, std::string& err) {
 }
 
 size_t Engine::engine_lds_bytes(const Variant& v) const {
-    return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap)) * 8;
+    return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap) + 32) * 8;
 }
 
 size_t Engine::plain_lds_bytes() const { return size_t(3) * g_.W * 8 + 258 * 4; }

@@ -390,7 +390,7 @@ __device__ __forceinline__ void make_features(const DevGraph& g, const StepCtx& 
 // W1 is stored input-major [16][64], W2 as [64 in][64 out], so every
 // weight load is one coalesced 256-byte row.  Returns argmin of the three
 // predicted log step costs among the allowed methods.
-__device__ __attribute__((noinline)) int mlp_choice(const float* w, const float* f, bool allow_as) {
+__device__ int mlp_choice(const float* w, const float* f, bool allow_as) {
     const int j = lane_id();
     float h = w[kMlpB1 + j];
 #pragma unroll
@@ -418,10 +418,12 @@ __device__ __attribute__((noinline)) int mlp_choice(const float* w, const float*
 }
 
 // ------------------------------------------------------------ engine
+constexpr int kPending = 8;  // queued decision-cache misses per wave
+
 template <int CAP>
 struct Lane {
-    // u64 words per wave: two bitmaps, two lists
-    static constexpr size_t words(int W) { return size_t(2) * W + CAP; }
+    // u64 words per wave: two bitmaps, two lists, the miss queue
+    static constexpr size_t words(int W) { return size_t(2) * W + CAP + kPending * 4; }
 };
 
 // Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
@@ -465,7 +467,8 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
 
 template <int CAP, int MODE>
 __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
-                            uint32_t* listA, uint32_t* listB, uint32_t& dcache, int* msteps) {
+                            uint32_t* listA, uint32_t* listB, uint32_t& dcache, int* pend, int& npend,
+                            int* msteps) {
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int L = int(a.offsets[sid + 1] - beg);
@@ -562,14 +565,19 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         const int c = int(win.at(b));
         RowSlice pair_next{0u, 0, 0};
         if (have_pair && b + 1 < L) pair_next = load_pair(c, int(win.at(b + 1)));
-        int n_tot = n_act;
-        if (imp >= 0) {
-            int ab, an;
-            alw_range(imp, ab, an);
-            n_tot += an;
-        } else if (imp == -2) {
-            n_tot += g.n_alw_states;
-        }
+        // active states including the implicit ones (selector features)
+        auto total_active = [&]() {
+            int n_tot = n_act;
+            if (imp >= 0) {
+                int ab, an;
+                alw_range(imp, ab, an);
+                n_tot += an;
+            } else if (imp == -2) {
+                n_tot += g.n_alw_states;
+            }
+            return n_tot;
+        };
+        const int n_tot = MODE == MODE_LEARNED ? n_act : total_active();
         ctx.c = c;
         ctx.n_act = n_tot;
         ctx.sparse = sparse;
@@ -594,10 +602,12 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             case MODE_LEARNED:
                 if (a.mlp == nullptr) {
                     load_stats();
+                    ctx.n_act = total_active();
                     m = heuristic_choice(g, ctx);
                 } else if (a.mlp_every > 0) {
                     // re-evaluate every mlp_every steps
                     load_stats();
+                    ctx.n_act = total_active();
                     if (cached < 0 || (b % a.mlp_every) == 0) {
                         float f[kFeatures];
                         make_features(g, ctx, f);
@@ -607,20 +617,27 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                     if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
                 } else {
                     // per-wave decision cache in one VGPR: byte c's entry is
-                    // byte (c & 3) of lane c >> 2, tagged with the active-count
-                    // bucket and list validity
-                    const int tag = ((31 - __clz(n_tot + 1)) << 1) | int(sparse);
+                    // byte (c & 3) of lane c >> 2, tagged with the bucket of
+                    // the listed active count and list validity.  A miss takes
+                    // the active-state method when the list is valid (iNFA
+                    // otherwise) and queues the step's features; the network
+                    // answers queued misses after the string.
+                    const int tag = ((31 - __clz(n_act + 1)) << 1) | int(sparse);
                     const int sh = (c & 3) * 8;
                     const int e = int((uint32_t(__builtin_amdgcn_readlane(int(dcache), c >> 2)) >> sh) & 0xffu);
                     if ((e >> 2) == tag) {
                         m = e & 3;
                     } else {
-                        load_stats();
-                        float f[kFeatures];
-                        make_features(g, ctx, f);
-                        m = mlp_choice(a.mlp, f, sparse);
+                        m = sparse ? M_AS : M_INFA;
                         if (lane_id() == (c >> 2))
                             dcache = (dcache & ~(0xffu << sh)) | (uint32_t((tag << 2) | m) << sh);
+                        if (npend < kPending) {
+                            const int l = lane_id();
+                            const int val = l == 0 ? c : l == 1 ? tag : l == 2 ? total_active() : l == 3 ? int(sparse)
+                                          : l == 4 ? b : l == 5 ? L : ctx.last_m;
+                            if (l < 7) pend[npend * 8 + l] = val;
+                            ++npend;
+                        }
                     }
                 }
                 break;
@@ -734,6 +751,31 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     wave_sync();
     clear_bits<CAP>(cur, cl, n_act, sparse, W);
     wave_sync();
+    if (MODE == MODE_LEARNED && a.mlp != nullptr) {
+        // answer the queued decision-cache misses with the network
+        for (int k = 0; k < npend; ++k) {
+            const int* q = pend + k * 8;
+            StepCtx qc;
+            qc.c = q[0];
+            qc.n_act = q[2];
+            qc.sparse = q[3];
+            qc.pos = q[4];
+            qc.len = q[5];
+            qc.last_m = q[6];
+            const int4 st = g.sym_stats[qc.c];
+            qc.T = st.x;
+            qc.K = st.y;
+            qc.Lft = st.z;
+            float f[kFeatures];
+            make_features(g, qc, f);
+            const int mm = mlp_choice(a.mlp, f, qc.sparse != 0);
+            const int sh = (qc.c & 3) * 8;
+            if (lane_id() == (qc.c >> 2) && int((dcache >> sh) & 0xffu) >> 2 == q[1])
+                dcache = (dcache & ~(0xffu << sh)) | (uint32_t((q[1] << 2) | mm) << sh);
+        }
+        npend = 0;
+        wave_sync();
+    }
 }
 
 template <int WPB, int CAP, int MODE>
@@ -752,6 +794,8 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
     uint32_t* listA = reinterpret_cast<uint32_t*>(base + 2 * g.W);
     uint32_t* listB = listA + CAP;
     uint32_t dcache = ~0u;  // learned selector: no cached decisions
+    int* pend = reinterpret_cast<int*>(base + 2 * g.W + CAP);
+    int npend = 0;
     for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
     wave_sync();
     int msteps[3] = {0, 0, 0};  // steps per method, this wave
@@ -761,7 +805,7 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
         if (idx >= a.n) break;
         const int sid = a.order ? a.order[idx] : idx;
-        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, dcache, msteps);
+        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, dcache, pend, npend, msteps);
     }
     if (a.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)
