@@ -1,11 +1,11 @@
 """Launch-variant autotuner for the scan engine.
 
-Sweeps the engine's waves per workgroup {1, 2, 4} and active-list capacity
-{256, 512, 1024} (the state words each lane owns follow from the rule set:
-ceil(W/64) u64 words per lane per bitmap), and the plain iNFA kernel's
-workgroup size {64, 128, 256}, on a device-resident sample of the
-workload; keeps the fastest by hipEvent kernel time.  Results are cached in
-admatch_amd/data/autotune.json keyed by the rule set's content hash.
+Sweeps the engine's state storage {LDS bitmaps, hashed lists with HBM
+bitmaps}, waves per workgroup {1, 2, 4} and active-list capacity
+{256, 512, 1024}, and the plain iNFA kernel's workgroup size {64, 128, 256},
+on a device-resident sample of the workload; keeps the fastest by hipEvent
+kernel time (all variants must give identical results).  Results are cached
+in admatch_amd/data/autotune.json keyed by the rule set's content hash.
 """
 from __future__ import annotations
 
@@ -36,36 +36,39 @@ def tune(s: E.Scratch, d_buf, d_off, n: int, mode: int = E.MODE_HEURISTIC, reps:
     info = s.info()
     results = []
     ref = None
-    for wpb in (1, 2, 4):
-        for cap in (256, 512, 1024):
-            try:
-                s.set_variant(wpb, cap, 64)
-                ms = _time(s, d_buf, d_off, n, E.KIND_ENGINE, mode, reps)
-            except E.AdmatchError as e:  # does not fit the LDS budget
-                log(f"  wpb={wpb} cap={cap}: skipped ({e})")
-                continue
-            out = s.collect(n)
-            if ref is None:
-                ref = out
-            elif not (np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])):
-                raise RuntimeError(f"variant wpb={wpb} cap={cap} changed the results")
-            log(f"  wpb={wpb} cap={cap}: {ms:.3f} ms")
-            results.append((ms, wpb, cap))
-    ms, wpb, cap = min(results)
-    best = {"wpb": wpb, "cap": cap, "engine_ms": ms, "mode": E.MODE_NAMES[mode], "states": info["states"],
-            "sweep": [{"wpb": w, "cap": c, "ms": m} for m, w, c in results]}
+    stores = (0, 1) if mode in (E.MODE_AS, E.MODE_LEARNED) else (0,)
+    for store in stores:
+        for wpb in (1, 2, 4):
+            for cap in (256, 512, 1024):
+                try:
+                    s.set_variant(wpb, cap, 64, store)
+                    ms = _time(s, d_buf, d_off, n, E.KIND_ENGINE, mode, reps)
+                except E.AdmatchError as e:  # does not fit the LDS budget
+                    log(f"  store={store} wpb={wpb} cap={cap}: skipped ({e})")
+                    continue
+                out = s.collect(n)
+                if ref is None:
+                    ref = out
+                elif not (np.array_equal(out[0], ref[0]) and np.array_equal(out[1], ref[1])):
+                    raise RuntimeError(f"variant store={store} wpb={wpb} cap={cap} changed the results")
+                log(f"  store={store} wpb={wpb} cap={cap}: {ms:.3f} ms")
+                results.append((ms, wpb, cap, store))
+    ms, wpb, cap, store = min(results)
+    best = {"wpb": wpb, "cap": cap, "store": store, "engine_ms": ms, "mode": E.MODE_NAMES[mode],
+            "states": info["states"],
+            "sweep": [{"wpb": w, "cap": c, "store": st, "ms": m} for m, w, c, st in results]}
     plain_bs = 64
     if plain and info["plain_lds_bytes"] <= 160 * 1024:
         pr = []
         for bs in (64, 128, 256):
-            s.set_variant(wpb, cap, bs)
+            s.set_variant(wpb, cap, bs, store)
             m = _time(s, d_buf, d_off, n, E.KIND_PLAIN, E.MODE_INFA, 1)
             log(f"  plain bs={bs}: {m:.3f} ms")
             pr.append((m, bs))
         plain_bs = min(pr)[1]
         best["plain_sweep"] = [{"bs": b, "ms": m} for m, b in pr]
     best["plain_bs"] = plain_bs
-    s.set_variant(wpb, cap, plain_bs)
+    s.set_variant(wpb, cap, plain_bs, store)
     return best
 
 
@@ -91,5 +94,5 @@ def apply(s: E.Scratch) -> bool:
     e = load(s.info()["hash"])
     if not e:
         return False
-    s.set_variant(e["wpb"], e["cap"], e.get("plain_bs", 64))
+    s.set_variant(e["wpb"], e["cap"], e.get("plain_bs", 64), e.get("store", 0))
     return True

@@ -54,6 +54,7 @@ struct ScanArgs {
     unsigned long long* log_top;
     int32_t log_stride;       // sample one step in log_stride
     unsigned long long* method_steps;  // [4] steps per method, or null
+    uint64_t* dense;          // hashed storage: two W-word bitmaps per wave of the grid, all zero
 };
 
 enum SelectMode : int32_t {

@@ -65,6 +65,8 @@ void Context::release() {
     dfree(d_order);
     dfree(d_text);
     dfree(d_off);
+    dfree(d_dense);
+    dense_words = 0;
     hfree(h_res_count);
     hfree(h_res_start);
     hfree(h_text);
@@ -76,8 +78,21 @@ void Context::release() {
     off_cap = 0;
 }
 
+int Context::ensure_dense(size_t words, std::string& err) {
+    if (words <= dense_words) return ADM_OK;
+    if (stream) ADM_HIP(hipStreamSynchronize(stream));
+    drop_graphs();  // captured launches hold the old buffer
+    dfree(d_dense);
+    dense_words = 0;
+    ADM_HIP(hipMalloc(&d_dense, words * 8));
+    ADM_HIP(hipMemset(d_dense, 0, words * 8));  // kernels keep it zero from here on
+    ADM_HIP(hipDeviceSynchronize());
+    dense_words = words;
+    return ADM_OK;
+}
+
 size_t Context::device_bytes() const {
-    return size_t(res_n_cap) * (4 + 8 + 4) + size_t(pool_cap) * 4 + size_t(text_cap) + size_t(off_cap) * 8 +
+    return dense_words * 8 + size_t(res_n_cap) * (4 + 8 + 4) + size_t(pool_cap) * 4 + size_t(text_cap) + size_t(off_cap) * 8 +
            kCtrWords * 4;
 }
 
@@ -157,7 +172,10 @@ int Engine::load(const char* path, std::string& err) {
     return ADM_OK;
 }
 
-size_t Engine::engine_lds_bytes(const Variant& v) const {
+size_t Engine::engine_lds_bytes(const Variant& v, int32_t mode) const {
+    // per wave: lists (cap u32 each) and the 8-record miss queue, plus either
+    // two W-word bitmaps or two hash tables of 2*cap u32
+    if (hashed(v, mode)) return size_t(v.wpb) * (size_t(3) * v.cap + 32) * 8;
     return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap) + 32) * 8;
 }
 
@@ -308,26 +326,39 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
         ADM_HIP(launch_plain(var_.plain_bs, n, lds, st, dev_, a));
         return ADM_OK;
     }
-    const size_t lds = engine_lds_bytes(var_);
-    if (lds + sizeof(int4) * 256 > max_lds_) {
-        err = "rule set too large for the engine's LDS budget (lower the waves per workgroup)";
-        return ADM_ERR_TOO_LARGE;
-    }
     if (mode == MODE_LOG && (!d_log_ || log_cap_ <= 0)) {
         err = "logging mode needs set_log() first";
         return ADM_ERR_STATE;
     }
+    int grid = 0;
+    if (int rc = engine_grid(mode, n, &grid, err)) return rc;
+    const bool hs = hashed(var_, mode);
+    if (hs) {
+        if (c.dense_words < size_t(grid) * var_.wpb * 2 * size_t(g_.W)) {
+            err = "dense bitmaps not prepared for this launch";
+            return ADM_ERR_STATE;
+        }
+        a.dense = c.d_dense;
+    }
     ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
+    ADM_HIP(launch_engine(var_.wpb, var_.cap, mode, hs, grid, engine_lds_bytes(var_, mode), st, dev_, a));
+    return ADM_OK;
+}
+
+int Engine::engine_grid(int32_t mode, int32_t n, int* grid, std::string& err) {
+    const size_t lds = engine_lds_bytes(var_, mode);
+    if (lds + sizeof(int4) * 256 > max_lds_) {
+        err = "rule set too large for the engine's LDS budget (lower the waves per workgroup)";
+        return ADM_ERR_TOO_LARGE;
+    }
     int blocks = 0;
-    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, mode, lds, &blocks));
+    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, mode, hashed(var_, mode), lds, &blocks));
     if (blocks <= 0) {
         err = "engine variant does not fit on a CU";
         return ADM_ERR_TOO_LARGE;
     }
-    int grid = blocks * cu_count_;
     const int need = (n + var_.wpb - 1) / var_.wpb;
-    grid = std::max(1, std::min(grid, need));
-    ADM_HIP(launch_engine(var_.wpb, var_.cap, mode, grid, lds, st, dev_, a));
+    *grid = std::max(1, std::min(blocks * cu_count_, need));
     return ADM_OK;
 }
 
@@ -343,6 +374,11 @@ int Engine::scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off,
     }
     if (int rc = c.init(err)) return rc;
     if (int rc = c.ensure_results(n, err)) return rc;
+    if (kind == KIND_ENGINE && n > 0 && hashed(var_, mode)) {
+        int grid = 0;
+        if (int rc = engine_grid(mode, n, &grid, err)) return rc;
+        if (int rc = c.ensure_dense(size_t(grid) * var_.wpb * 2 * size_t(g_.W), err)) return rc;
+    }
     hipStream_t st = c.stream;
     ADM_HIP(hipEventRecord(c.ev0, st));
     if (use_graph && n > 0) {

@@ -36,6 +36,10 @@ struct Variant {
     int wpb = 2;        // waves (strings) per workgroup
     int cap = 512;      // active-list capacity per string
     int plain_bs = 64;  // workgroup size of the plain iNFA kernel
+    // engine storage for the AS and learned modes: 0 = two S-bit bitmaps per
+    // string in LDS; 1 = active lists with LDS hash tables, the bitmaps in
+    // HBM (ScanArgs::dense).  The other modes always use LDS bitmaps.
+    int store = 0;
 };
 
 struct ScanStats {
@@ -67,6 +71,8 @@ struct Context {
     int64_t* h_off = nullptr;
     int64_t* d_off = nullptr;
     int32_t off_cap = 0;
+    uint64_t* d_dense = nullptr;  // hashed storage bitmaps (zero between launches)
+    size_t dense_words = 0;
     std::map<std::tuple<const void*, const void*, int32_t, int32_t, int32_t>, hipGraphExec_t> graphs;
     ScanStats stats;
 
@@ -77,6 +83,7 @@ struct Context {
     int ensure_results(int32_t n, std::string& err);
     int ensure_staging(int64_t bytes, int32_t n, std::string& err);
     int grow_pool(std::string& err);
+    int ensure_dense(size_t words, std::string& err);
     size_t device_bytes() const;
 };
 
@@ -132,12 +139,16 @@ class Engine {
     // step logging for the learned selector (MODE_LOG)
     int set_log(int64_t cap_records, int32_t stride, std::string& err);
     int64_t fetch_log(float* out, int64_t cap_records, std::string& err);
-    size_t engine_lds_bytes(const Variant& v) const;
+    size_t engine_lds_bytes(const Variant& v, int32_t mode = MODE_LEARNED) const;
+    bool hashed(const Variant& v, int32_t mode) const {
+        return v.store == 1 && (mode == MODE_AS || mode == MODE_LEARNED);
+    }
     size_t plain_lds_bytes() const;
     int device_cu_count() const { return cu_count_; }
     size_t table_bytes() const { return table_bytes_; }
 
    private:
+    int engine_grid(int32_t mode, int32_t n, int* grid, std::string& err);
     int launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
                std::string& err);
     ScanArgs make_args(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, int32_t mode);

@@ -22,11 +22,8 @@ namespace admatch {
 
 // ------------------------------------------------------------------ common
 // List entries carry bit 31 for always-active states (their rows come from
-// the per-byte always-active list instead of the AS table) and bit 30 for
-// first positions added by the always-active rows of the previous byte
-// (their rows come from the two-byte rows).
+// the per-byte always-active list instead of the AS table).
 constexpr uint32_t kTag = 0x80000000u;
-constexpr uint32_t kFirst = 0x40000000u;
 constexpr uint32_t kIdMask = 0x00ffffffu;
 
 __device__ __forceinline__ bool set_new(uint64_t* bits, uint32_t d) {
@@ -38,6 +35,52 @@ __device__ __forceinline__ bool set_new(uint64_t* bits, uint32_t d) {
 __device__ __forceinline__ void set_bit_atomic(uint64_t* bits, uint32_t d) {
     atomicOr(reinterpret_cast<unsigned long long*>(&bits[d >> 6]), 1ull << (d & 63));
 }
+
+// Destination sets of a step.  BitSet: a bitmap of all S states.  HashSet: an
+// open-addressing table of H slots in LDS (linear probing, kEmpty marks a free
+// slot); it holds at most CAP + 64 keys, so steps that use it stop early
+// (kBounded) once the list passes CAP and are redone on the bitmaps.
+constexpr uint32_t kEmpty = 0xffffffffu;
+
+struct BitSet {
+    static constexpr bool kBounded = false;
+    uint64_t* bits;
+    __device__ __forceinline__ bool insert(uint32_t d) const { return set_new(bits, d); }
+};
+
+template <int H>
+struct HashSet {
+    static_assert((H & (H - 1)) == 0, "table size must be a power of two");
+    static constexpr bool kBounded = true;
+    uint32_t* tab;
+    __device__ __forceinline__ static uint32_t slot(uint32_t d) {
+        return (d * 2654435761u) >> (32 - __builtin_ctz(uint32_t(H)));
+    }
+    // true if d was not in the set
+    __device__ __forceinline__ bool insert(uint32_t d) const {
+        uint32_t i = slot(d);
+        for (;;) {
+            const uint32_t old = atomicCAS(&tab[i], kEmpty, d);
+            if (old == kEmpty) return true;
+            if (old == d) return false;
+            i = (i + 1) & uint32_t(H - 1);
+        }
+    }
+    __device__ __forceinline__ bool contains(uint32_t d) const {
+        uint32_t i = slot(d);
+        for (;;) {
+            const uint32_t v = tab[i];
+            if (v == d) return true;
+            if (v == kEmpty) return false;
+            i = (i + 1) & uint32_t(H - 1);
+        }
+    }
+    // every slot free again (whole wave)
+    __device__ __forceinline__ void clear() const {
+        uint4* t4 = reinterpret_cast<uint4*>(tab);
+        for (int j = lane_id(); j < H / 4; j += 64) t4[j] = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+    }
+};
 
 // Byte i of a string, read through a 256-byte window held one u32 per lane.
 struct ByteWindow {
@@ -261,8 +304,8 @@ __device__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64
 // Flattened processing of multi-destination rows: lane l owns a row of
 // len_l destinations starting at as_multi[start_l]; each iteration every
 // lane fetches one destination (coalesced within a row).
-template <int CAP>
-__device__ int multi_rows(const DevGraph& g, uint32_t start, int len, uint64_t* fut, uint32_t* fl, int n) {
+template <int CAP, typename SET>
+__device__ int multi_rows(const DevGraph& g, uint32_t start, int len, const SET& fut, uint32_t* fl, int n) {
     const int incl = wave_scan(len);
     const int excl = incl - len;
     const int tot = uniform(__shfl(incl, 63, 64));
@@ -280,8 +323,9 @@ __device__ int multi_rows(const DevGraph& g, uint32_t start, int len, uint64_t* 
         const int exo = __shfl(excl, lo, 64);
         const bool ok = t < tot;
         const uint32_t d = ok ? g.as_multi[so + uint32_t(t - exo)] : 0u;
-        const bool nw = ok && set_new(fut, d);
+        const bool nw = ok && fut.insert(d);
         n = list_append<CAP>(fl, n, nw, d);
+        if (SET::kBounded && n > CAP) return n;
     }
     return n;
 }
@@ -301,9 +345,9 @@ struct RowSlice {
 // here when use_pair) and the always-active rows of c, which become the
 // next step's implicit set.  Always-active states (bit 31) are therefore
 // neither gathered nor stored.
-template <int CAP>
+template <int CAP, typename SET>
 __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_pair, const uint32_t* cl, int n_cur,
-                       uint64_t* fut, uint32_t* fl) {
+                       const SET& fut, uint32_t* fl) {
     const uint32_t* tab = g.as_tab + size_t(c) * g.S;
     // the first chunk's table gathers are in flight while the two-byte rows
     // are merged
@@ -315,8 +359,9 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
             const int t = t0 + lane_id();
             const bool ok = t < pair.n;
             const uint32_t v = t0 == 0 ? pair.v : (ok ? g.pair_dst[pair.b + t] : 0u);
-            const bool nw = ok && set_new(fut, v);
+            const bool nw = ok && fut.insert(v);
             n = list_append<CAP>(fl, n, nw, v);
+            if (SET::kBounded && n > CAP) return n;
         }
     }
     for (int k0 = 0; k0 < n_cur; k0 += 64) {
@@ -328,8 +373,9 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
         }
         const uint32_t cnt = e >> 24;
         const uint32_t d = e & kIdMask;
-        const bool nw = cnt == 1 && set_new(fut, d);
+        const bool nw = cnt == 1 && fut.insert(d);
         n = list_append<CAP>(fl, n, nw, d);
+        if (SET::kBounded && n > CAP) return n;
         // short rows: each lane walks its own row, four loads in flight
         const int len = (cnt >= 2 && cnt < 255) ? int(cnt) : 0;
         if (__ballot(len > 0)) {
@@ -338,18 +384,21 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
             for (int j = 0; j < 4; ++j) x[j] = j < len ? g.as_multi[d + j] : 0u;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const bool nj = j < len && set_new(fut, x[j]);
+                const bool nj = j < len && fut.insert(x[j]);
                 n = list_append<CAP>(fl, n, nj, x[j]);
+                if (SET::kBounded && n > CAP) return n;
             }
             for (int j = 4; __ballot(j < len); ++j) {
                 const uint32_t y = j < len ? g.as_multi[d + j] : 0u;
-                const bool nj = j < len && set_new(fut, y);
+                const bool nj = j < len && fut.insert(y);
                 n = list_append<CAP>(fl, n, nj, y);
+                if (SET::kBounded && n > CAP) return n;
             }
         }
         // long rows (255 or more destinations): flattened over the wave
         const bool lng = cnt == 255;
         if (__ballot(lng)) n = multi_rows<CAP>(g, d + 1, lng ? int(g.as_multi[d]) : 0, fut, fl, n);
+        if (SET::kBounded && n > CAP) return n;
     }
     return n;
 }
@@ -417,8 +466,123 @@ __device__ int mlp_choice(const float* w, const float* f, bool allow_as) {
     return arg;
 }
 
+
+// Per-wave state of the learned selector.  dcache holds one decision per
+// byte value: byte c's entry is byte (c & 3) of lane c >> 2, tagged with the
+// bucket of the listed active count and list validity.  pend queues the
+// features of up to kPending cache misses; the network answers them after
+// the string (answer_pending), outside the step loop.
+constexpr int kPending = 8;
+
+struct Selector {
+    uint32_t dcache;
+    int* pend;  // kPending records of 8 ints in LDS
+    int npend;
+    int cached;  // mlp_every > 0: the last evaluated decision
+};
+
+// Method for the step on ctx.c.  total_active() counts the active states
+// including the implicitly held ones (selector features).
+template <int MODE, typename TOTAL>
+__device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& a, int sid, StepCtx& ctx, int n_act,
+                                             bool sparse, const TOTAL& total_active, Selector& sel, bool& log_step) {
+    const int c = ctx.c;
+    const int b = ctx.pos;
+    ctx.n_act = MODE == MODE_LEARNED ? n_act : total_active();
+    // per-byte statistics, read only by the selectors that use them
+    auto load_stats = [&]() {
+        const int4 st = g.sym_stats[c];
+        ctx.T = st.x;
+        ctx.K = st.y;
+        ctx.Lft = st.z;
+    };
+    int m;
+    switch (MODE) {
+        case MODE_INFA: m = M_INFA; break;
+        case MODE_TKO: m = M_TKO; break;
+        case MODE_AS: m = sparse ? M_AS : M_INFA; break;
+        case MODE_ADMATCH:
+            load_stats();
+            m = (sparse && ctx.K > ctx.n_act) ? M_AS : M_TKO;
+            break;
+        case MODE_LEARNED:
+            if (a.mlp == nullptr) {
+                load_stats();
+                ctx.n_act = total_active();
+                m = heuristic_choice(g, ctx);
+            } else if (a.mlp_every > 0) {
+                // re-evaluate every mlp_every steps
+                load_stats();
+                ctx.n_act = total_active();
+                if (sel.cached < 0 || (b % a.mlp_every) == 0) {
+                    float f[kFeatures];
+                    make_features(g, ctx, f);
+                    sel.cached = mlp_choice(a.mlp, f, true);
+                }
+                m = sel.cached;
+                if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
+            } else {
+                // decision cache; a miss takes the active-state method when
+                // the list is valid (iNFA otherwise) and queues the features
+                const int tag = ((31 - __clz(n_act + 1)) << 1) | int(sparse);
+                const int sh = (c & 3) * 8;
+                const int e = int((uint32_t(__builtin_amdgcn_readlane(int(sel.dcache), c >> 2)) >> sh) & 0xffu);
+                if ((e >> 2) == tag) {
+                    m = e & 3;
+                } else {
+                    m = sparse ? M_AS : M_INFA;
+                    if (lane_id() == (c >> 2))
+                        sel.dcache = (sel.dcache & ~(0xffu << sh)) | (uint32_t((tag << 2) | m) << sh);
+                    if (sel.npend < kPending) {
+                        const int l = lane_id();
+                        const int val = l == 0 ? c : l == 1 ? tag : l == 2 ? total_active() : l == 3 ? int(sparse)
+                                      : l == 4 ? b : l == 5 ? ctx.len : ctx.last_m;
+                        if (l < 7) sel.pend[sel.npend * 8 + l] = val;
+                        ++sel.npend;
+                    }
+                }
+            }
+            break;
+        case MODE_LOG:
+            load_stats();
+            m = heuristic_choice(g, ctx);
+            log_step = ((uint64_t(sid) * 2654435761ull + uint64_t(b)) % uint64_t(a.log_stride)) == 0;
+            break;
+        default:
+            load_stats();
+            m = heuristic_choice(g, ctx);
+            break;
+    }
+    return m;
+}
+
+// Answers the queued decision-cache misses with the network (whole wave).
+__device__ void answer_pending(const DevGraph& g, const ScanArgs& a, Selector& sel) {
+    for (int k = 0; k < sel.npend; ++k) {
+        const int* q = sel.pend + k * 8;
+        StepCtx qc;
+        qc.c = q[0];
+        qc.n_act = q[2];
+        qc.sparse = q[3];
+        qc.pos = q[4];
+        qc.len = q[5];
+        qc.last_m = q[6];
+        const int4 st = g.sym_stats[qc.c];
+        qc.T = st.x;
+        qc.K = st.y;
+        qc.Lft = st.z;
+        float f[kFeatures];
+        make_features(g, qc, f);
+        const int mm = mlp_choice(a.mlp, f, qc.sparse != 0);
+        const int sh = (qc.c & 3) * 8;
+        if (lane_id() == (qc.c >> 2) && int((sel.dcache >> sh) & 0xffu) >> 2 == q[1])
+            sel.dcache = (sel.dcache & ~(0xffu << sh)) | (uint32_t((q[1] << 2) | mm) << sh);
+    }
+    sel.npend = 0;
+    wave_sync();
+}
+
 // ------------------------------------------------------------ engine
-constexpr int kPending = 8;  // queued decision-cache misses per wave
 
 template <int CAP>
 struct Lane {
@@ -467,8 +631,7 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
 
 template <int CAP, int MODE>
 __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
-                            uint32_t* listA, uint32_t* listB, uint32_t& dcache, int* pend, int& npend,
-                            int* msteps) {
+                            uint32_t* listA, uint32_t* listB, Selector& sel, int* msteps) {
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int L = int(a.offsets[sid + 1] - beg);
@@ -560,7 +723,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     ctx.len = L;
     ctx.last_m = M_INFA;
     ctx.c_prev = -1;
-    int cached = -1;
+    sel.cached = -1;
     for (int b = 0; b < L; ++b) {
         const int c = int(win.at(b));
         RowSlice pair_next{0u, 0, 0};
@@ -577,87 +740,18 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             }
             return n_tot;
         };
-        const int n_tot = MODE == MODE_LEARNED ? n_act : total_active();
         ctx.c = c;
-        ctx.n_act = n_tot;
         ctx.sparse = sparse;
         ctx.pos = b;
-        // per-byte statistics, read only by the selectors that use them
-        auto load_stats = [&]() {
-            const int4 st = g.sym_stats[c];
-            ctx.T = st.x;
-            ctx.K = st.y;
-            ctx.Lft = st.z;
-        };
-        int m;
         bool log_step = false;
-        switch (MODE) {
-            case MODE_INFA: m = M_INFA; break;
-            case MODE_TKO: m = M_TKO; break;
-            case MODE_AS: m = sparse ? M_AS : M_INFA; break;
-            case MODE_ADMATCH:
-                load_stats();
-                m = (sparse && ctx.K > n_tot) ? M_AS : M_TKO;
-                break;
-            case MODE_LEARNED:
-                if (a.mlp == nullptr) {
-                    load_stats();
-                    ctx.n_act = total_active();
-                    m = heuristic_choice(g, ctx);
-                } else if (a.mlp_every > 0) {
-                    // re-evaluate every mlp_every steps
-                    load_stats();
-                    ctx.n_act = total_active();
-                    if (cached < 0 || (b % a.mlp_every) == 0) {
-                        float f[kFeatures];
-                        make_features(g, ctx, f);
-                        cached = mlp_choice(a.mlp, f, true);
-                    }
-                    m = cached;
-                    if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
-                } else {
-                    // per-wave decision cache in one VGPR: byte c's entry is
-                    // byte (c & 3) of lane c >> 2, tagged with the bucket of
-                    // the listed active count and list validity.  A miss takes
-                    // the active-state method when the list is valid (iNFA
-                    // otherwise) and queues the step's features; the network
-                    // answers queued misses after the string.
-                    const int tag = ((31 - __clz(n_act + 1)) << 1) | int(sparse);
-                    const int sh = (c & 3) * 8;
-                    const int e = int((uint32_t(__builtin_amdgcn_readlane(int(dcache), c >> 2)) >> sh) & 0xffu);
-                    if ((e >> 2) == tag) {
-                        m = e & 3;
-                    } else {
-                        m = sparse ? M_AS : M_INFA;
-                        if (lane_id() == (c >> 2))
-                            dcache = (dcache & ~(0xffu << sh)) | (uint32_t((tag << 2) | m) << sh);
-                        if (npend < kPending) {
-                            const int l = lane_id();
-                            const int val = l == 0 ? c : l == 1 ? tag : l == 2 ? total_active() : l == 3 ? int(sparse)
-                                          : l == 4 ? b : l == 5 ? L : ctx.last_m;
-                            if (l < 7) pend[npend * 8 + l] = val;
-                            ++npend;
-                        }
-                    }
-                }
-                break;
-            case MODE_LOG:
-                load_stats();
-                m = heuristic_choice(g, ctx);
-                log_step = ((uint64_t(sid) * 2654435761ull + uint64_t(b)) % uint64_t(a.log_stride)) == 0;
-                break;
-            default:
-                load_stats();
-                m = heuristic_choice(g, ctx);
-                break;
-        }
+        int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, sel, log_step);
         if (m == M_AS && !(sparse && can_as)) m = M_INFA;
         bool fsp;
         int n_fut;
         int next_imp = -1;
         if (!log_step) {
             if (m == M_AS) {
-                n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, fut, fl);
+                n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, BitSet{fut}, fl);
                 fsp = n_fut <= CAP;
                 next_imp = c;
             } else {
@@ -671,7 +765,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             if (sparse && can_as) {
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, fut, fl);
+                n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, BitSet{fut}, fl);
                 fsp = n_fut <= CAP;
                 wave_sync();
                 __builtin_amdgcn_s_waitcnt(0);
@@ -751,31 +845,308 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     wave_sync();
     clear_bits<CAP>(cur, cl, n_act, sparse, W);
     wave_sync();
-    if (MODE == MODE_LEARNED && a.mlp != nullptr) {
-        // answer the queued decision-cache misses with the network
-        for (int k = 0; k < npend; ++k) {
-            const int* q = pend + k * 8;
-            StepCtx qc;
-            qc.c = q[0];
-            qc.n_act = q[2];
-            qc.sparse = q[3];
-            qc.pos = q[4];
-            qc.len = q[5];
-            qc.last_m = q[6];
-            const int4 st = g.sym_stats[qc.c];
-            qc.T = st.x;
-            qc.K = st.y;
-            qc.Lft = st.z;
-            float f[kFeatures];
-            make_features(g, qc, f);
-            const int mm = mlp_choice(a.mlp, f, qc.sparse != 0);
-            const int sh = (qc.c & 3) * 8;
-            if (lane_id() == (qc.c >> 2) && int((dcache >> sh) & 0xffu) >> 2 == q[1])
-                dcache = (dcache & ~(0xffu << sh)) | (uint32_t((q[1] << 2) | mm) << sh);
-        }
-        npend = 0;
-        wave_sync();
+    if (MODE == MODE_LEARNED && a.mlp != nullptr) answer_pending(g, a, sel);
+}
+
+// ------------------------------------------------------------ hashed storage
+// The same step loop with the set held as the active list plus an LDS hash
+// table of 2*CAP slots per list (instead of two S-bit bitmaps), so a wave's
+// LDS no longer grows with the rule set and many more waves fit on a CU.
+// Two S-bit bitmaps per wave live in HBM (ScanArgs::dense) and are touched
+// only by dense steps: a set of more than CAP states, a step whose method is
+// iNFA or TKO, an AS step that overflowed the list (redone as iNFA), and
+// acceptance.  Invariants between steps: the fut table is empty and the fut
+// bitmap is zero; a sparse set has its states in cl and hc (cur bitmap zero),
+// a dense set in the cur bitmap (hc empty).
+
+// Orders this wave's global-memory accesses across lanes (the HBM bitmaps).
+__device__ __forceinline__ void dense_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ int count_bits(const uint64_t* bits, int W) {
+    int n = 0;
+    for (int j = lane_id(); j < W; j += 64) n += __popcll(bits[j]);
+    return uniform(wave_sum(n));
+}
+
+// iNFA or TKO step on whole bitmaps (fut zero on entry).
+__device__ void dense_method(const DevGraph& g, int m, int c, const uint64_t* cur, uint64_t* fut) {
+    if (m == M_TKO) {
+        tko_groups(g, c, cur, fut);
+        dense_sync();
+        scatter<64, false>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, nullptr, 0);
+    } else {
+        for (int j = lane_id(); j < g.W; j += 64) fut[j] = cur[j] & g.persis[j];
+        dense_sync();
+        scatter<64, false>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, nullptr, 0);
     }
+    dense_sync();
+}
+
+template <int CAP, int MODE>
+__device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* gA, uint64_t* gB,
+                              uint32_t* hA, uint32_t* hB, uint32_t* listA, uint32_t* listB, Selector& sel,
+                              int* msteps) {
+    using HS = HashSet<2 * CAP>;
+    const int W = g.W;
+    const int64_t beg = a.offsets[sid];
+    const int L = int(a.offsets[sid + 1] - beg);
+    uint64_t* gcur = gA;
+    uint64_t* gfut = gB;
+    HS hc{hA}, hf{hB};
+    uint32_t* cl = listA;
+    uint32_t* fl = listB;
+    // the always-active states start implicit, the other initial states listed
+    int n_act = uniform(rebuild_list<CAP>(g.init_explicit, W, cl));
+    bool sparse = n_act <= CAP;
+    if (sparse) {
+        for (int k = lane_id(); k < n_act; k += 64) hc.insert(cl[k]);
+    } else {
+        for (int j = lane_id(); j < W; j += 64) gcur[j] = g.init_explicit[j];
+        dense_sync();
+    }
+    wave_sync();
+    ByteWindow win{a.text + beg, L, 0, 0u};
+    if (L > 0) win.load(0);
+    const bool has_wb = g.wb_count > 0;
+    const bool have_pair = g.pair_off != nullptr;
+    int imp = -2;  // implicitly held states, as in scan_string
+    auto alw_range = [&](int cc, int& b0, int& n0) {
+        b0 = g.sym_stats[cc].w;
+        n0 = (cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total) - b0;
+    };
+    // sparse -> dense: the listed states into the cur bitmap
+    auto to_dense = [&]() {
+        if (!sparse) return;
+        for (int k = lane_id(); k < n_act; k += 64) set_bit_atomic(gcur, cl[k] & kIdMask);
+        hc.clear();
+        sparse = false;
+        dense_sync();
+        wave_sync();
+    };
+    auto materialize = [&]() {
+        if (imp == -1) return;
+        int ab = 0, an = g.n_alw_states;
+        const uint32_t* src = g.alw_states;
+        if (imp >= 0) {
+            alw_range(imp, ab, an);
+            src = g.alw_dst;
+        }
+        if (sparse && n_act + an <= CAP) {
+            for (int t0 = 0; t0 < an; t0 += 64) {
+                const int t = t0 + lane_id();
+                const bool ok = t < an;
+                const uint32_t v = ok ? (imp >= 0 ? src[ab + t] : (src[t] | kTag)) : 0u;
+                const bool nw = ok && hc.insert(v & kIdMask);
+                n_act = list_append<CAP>(cl, n_act, nw, v & (kIdMask | kTag));
+            }
+            n_act = uniform(n_act);
+            wave_sync();
+        } else {
+            to_dense();
+            for (int t = lane_id(); t < an; t += 64) set_bit_atomic(gcur, src[ab + t] & kIdMask);
+            dense_sync();
+            n_act = count_bits(gcur, W);
+        }
+        imp = -1;
+    };
+    auto wb_pass = [&]() {
+        // in place, list order, one lane (thread_count == 1 semantics of
+        // src/iNFA_kernel.cu:145-164)
+        materialize();
+        if (sparse && n_act + g.wb_count <= CAP) {
+            int n = n_act;
+            if (lane_id() == 0) {
+                for (int t = 0; t < g.wb_count; ++t) {
+                    const uint2 e = g.list[g.wb_begin + t];
+                    if (hc.contains(e.x) && hc.insert(e.y)) cl[n++] = e.y;
+                }
+            }
+            n_act = uniform(__builtin_amdgcn_readlane(n, 0));
+            wave_sync();
+        } else {
+            to_dense();
+            if (lane_id() == 0) {
+                for (int t = 0; t < g.wb_count; ++t) {
+                    const uint2 e = g.list[g.wb_begin + t];
+                    if (test_bit(gcur, e.x)) gcur[e.y >> 6] |= 1ull << (e.y & 63);
+                }
+            }
+            dense_sync();
+            n_act = count_bits(gcur, W);
+        }
+    };
+    if (has_wb && L > 0 && word_char(win.at(0))) wb_pass();
+
+    auto load_pair = [&](int c1, int c2) -> RowSlice {
+        RowSlice r;
+        const uint32_t* o = g.pair_off + (c1 * 256 + c2);
+        r.b = int(o[0]);
+        r.n = int(o[1]) - r.b;
+        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
+        return r;
+    };
+    RowSlice pair{0u, 0, 0};
+    const bool can_as = g.as_tab != nullptr;
+
+    StepCtx ctx;
+    ctx.len = L;
+    ctx.last_m = M_INFA;
+    ctx.c_prev = -1;
+    sel.cached = -1;
+    for (int b = 0; b < L; ++b) {
+        const int c = int(win.at(b));
+        RowSlice pair_next{0u, 0, 0};
+        if (have_pair && b + 1 < L) pair_next = load_pair(c, int(win.at(b + 1)));
+        auto total_active = [&]() {
+            int n_tot = n_act;
+            if (imp >= 0) {
+                int ab, an;
+                alw_range(imp, ab, an);
+                n_tot += an;
+            } else if (imp == -2) {
+                n_tot += g.n_alw_states;
+            }
+            return n_tot;
+        };
+        ctx.c = c;
+        ctx.sparse = sparse;
+        ctx.pos = b;
+        bool log_step = false;
+        int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, sel, log_step);
+        if (m == M_AS && !(sparse && can_as)) m = M_INFA;
+        int n_fut = 0;
+        bool fsp = false;
+        if (m == M_AS) {
+            n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, hf, fl);
+            wave_sync();
+            if (n_fut <= CAP) {
+                fsp = true;
+            } else {
+                // the list overflowed: drop the partial result, redo as iNFA
+                hf.clear();
+                wave_sync();
+                m = M_INFA;
+            }
+        }
+        if (fsp) {
+            hc.clear();  // retire cur (its bitmap is zero)
+            HS th = hc;
+            hc = hf;
+            hf = th;
+            uint32_t* tl = cl;
+            cl = fl;
+            fl = tl;
+            n_act = n_fut;
+            imp = c;
+            wave_sync();
+        } else {
+            materialize();
+            to_dense();
+            dense_method(g, m, c, gcur, gfut);
+            for (int j = lane_id(); j < W; j += 64) gcur[j] = 0ull;  // retire cur
+            uint64_t* tb = gcur;
+            gcur = gfut;
+            gfut = tb;
+            dense_sync();
+            n_act = uniform(rebuild_list<CAP>(gcur, W, cl));
+            if (n_act <= CAP) {
+                // back to the list form
+                for (int k = lane_id(); k < n_act; k += 64) {
+                    hc.insert(cl[k]);
+                    gcur[cl[k] >> 6] = 0ull;
+                }
+                sparse = true;
+                dense_sync();
+            }
+            wave_sync();
+            imp = -1;
+        }
+        ctx.c_prev = c;
+        msteps[m] += 1;
+        ctx.last_m = m;
+        if (has_wb) {
+            const uint32_t xb = uint32_t(c);
+            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(win.at(b + 1))) : word_char(xb);
+            if (boundary) wb_pass();
+        }
+        pair = pair_next;
+    }
+    materialize();
+    if (sparse) {
+        // accepted listed states into the (zero) fut bitmap, emitted from there
+        // in ascending order
+        int cnt = 0;
+        for (int k = lane_id(); k < n_act; k += 64) {
+            const uint32_t id = cl[k] & kIdMask;
+            if (test_bit(g.accept, id)) {
+                set_bit_atomic(gfut, id);
+                ++cnt;
+            }
+        }
+        cnt = uniform(wave_sum(cnt));
+        if (cnt == 0) {
+            if (lane_id() == 0) {
+                a.res_count[sid] = 0;
+                a.res_start[sid] = 0;
+            }
+        } else {
+            dense_sync();
+            emit_accepts(g, a, sid, gfut);
+            dense_sync();
+            for (int k = lane_id(); k < n_act; k += 64) gfut[(cl[k] & kIdMask) >> 6] = 0ull;
+            dense_sync();
+        }
+        hc.clear();
+    } else {
+        emit_accepts(g, a, sid, gcur);
+        dense_sync();
+        for (int j = lane_id(); j < W; j += 64) gcur[j] = 0ull;
+        dense_sync();
+    }
+    wave_sync();
+    if (MODE == MODE_LEARNED && a.mlp != nullptr) answer_pending(g, a, sel);
+}
+
+template <int CAP>
+struct LaneH {
+    // u64 words per wave: two lists of CAP u32, two tables of 2*CAP u32, the miss queue
+    static constexpr size_t words() { return size_t(3) * CAP + kPending * 4; }
+};
+
+template <int WPB, int CAP, int MODE>
+__global__ __launch_bounds__(64 * WPB) void engine_h_kernel(DevGraph g, ScanArgs a) {
+    extern __shared__ uint64_t lds[];
+    __shared__ int4 s_stats[256];
+    for (int i = threadIdx.x; i < 256; i += 64 * WPB) s_stats[i] = g.sym_stats[i];
+    __syncthreads();
+    g.sym_stats = s_stats;
+    const int wave = threadIdx.x >> 6;
+    uint64_t* base = lds + LaneH<CAP>::words() * wave;
+    uint32_t* listA = reinterpret_cast<uint32_t*>(base);
+    uint32_t* listB = listA + CAP;
+    uint32_t* hA = listB + CAP;
+    uint32_t* hB = hA + 2 * CAP;
+    Selector sel{~0u, reinterpret_cast<int*>(hB + 2 * CAP), 0, -1};
+    uint64_t* gA = a.dense + (size_t(blockIdx.x) * WPB + wave) * 2 * size_t(g.W);
+    uint64_t* gB = gA + g.W;
+    HashSet<2 * CAP>{hA}.clear();
+    HashSet<2 * CAP>{hB}.clear();
+    wave_sync();
+    int msteps[3] = {0, 0, 0};
+    for (;;) {
+        int idx = 0;
+        if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
+        idx = uniform(__builtin_amdgcn_readlane(idx, 0));
+        if (idx >= a.n) break;
+        const int sid = a.order ? a.order[idx] : idx;
+        scan_string_h<CAP, MODE>(g, a, sid, gA, gB, hA, hB, listA, listB, sel, msteps);
+    }
+    if (a.method_steps != nullptr && lane_id() == 0)
+        for (int m = 0; m < 3; ++m)
+            if (msteps[m]) atomicAdd(&a.method_steps[m], (unsigned long long)msteps[m]);
 }
 
 template <int WPB, int CAP, int MODE>
@@ -793,9 +1164,7 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
     uint64_t* bitsB = base + g.W;
     uint32_t* listA = reinterpret_cast<uint32_t*>(base + 2 * g.W);
     uint32_t* listB = listA + CAP;
-    uint32_t dcache = ~0u;  // learned selector: no cached decisions
-    int* pend = reinterpret_cast<int*>(base + 2 * g.W + CAP);
-    int npend = 0;
+    Selector sel{~0u, reinterpret_cast<int*>(base + 2 * g.W + CAP), 0, -1};  // no cached decisions
     for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
     wave_sync();
     int msteps[3] = {0, 0, 0};  // steps per method, this wave
@@ -805,7 +1174,7 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
         if (idx >= a.n) break;
         const int sid = a.order ? a.order[idx] : idx;
-        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, dcache, pend, npend, msteps);
+        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, sel, msteps);
     }
     if (a.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)
@@ -856,42 +1225,58 @@ namespace {
 // per-workgroup byte statistics
 constexpr int kEngineDynLds = 160 * 1024 - int(sizeof(int4)) * 256;
 
-template <int WPB, int CAP, int MODE>
+template <int WPB, int CAP, int MODE, bool HASHED>
+constexpr auto engine_fn() {
+    if constexpr (HASHED) return &engine_h_kernel<WPB, CAP, MODE>;
+    else return &engine_kernel<WPB, CAP, MODE>;
+}
+template <int WPB, int CAP, int MODE, bool HASHED>
 hipError_t engine_launch_m(int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
+    constexpr auto fn = engine_fn<WPB, CAP, MODE, HASHED>();
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP, MODE>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, kEngineDynLds);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  kEngineDynLds);
         (void)hipGetLastError();  // an unsupported attribute must not fail the launch check
         attr = true;
     }
-    hipLaunchKernelGGL((engine_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * WPB), lds, st, g, a);
     return hipGetLastError();
 }
-template <int WPB, int CAP, int MODE>
+template <int WPB, int CAP, int MODE, bool HASHED>
 hipError_t engine_occ_m(size_t lds, int* blocks) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&engine_kernel<WPB, CAP, MODE>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, kEngineDynLds);
+    constexpr auto fn = engine_fn<WPB, CAP, MODE, HASHED>();
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              kEngineDynLds);
     (void)hipGetLastError();
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, engine_kernel<WPB, CAP, MODE>, 64 * WPB, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, fn, 64 * WPB, lds);
 }
-#define ADM_DISPATCH_MODE(FN, ...)                                         \
-    switch (mode) {                                                        \
-        case MODE_INFA: return FN<WPB, CAP, MODE_INFA>(__VA_ARGS__);       \
-        case MODE_TKO: return FN<WPB, CAP, MODE_TKO>(__VA_ARGS__);         \
-        case MODE_AS: return FN<WPB, CAP, MODE_AS>(__VA_ARGS__);           \
-        case MODE_ADMATCH: return FN<WPB, CAP, MODE_ADMATCH>(__VA_ARGS__); \
-        case MODE_HEUR: return FN<WPB, CAP, MODE_HEUR>(__VA_ARGS__);       \
-        case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED>(__VA_ARGS__); \
-        case MODE_LOG: return FN<WPB, CAP, MODE_LOG>(__VA_ARGS__);         \
-        default: return hipErrorInvalidValue;                              \
+// hashed storage is built for the modes that run mostly AS steps
+#define ADM_DISPATCH_MODE(FN, ...)                                                      \
+    if (hashed) {                                                                       \
+        switch (mode) {                                                                 \
+            case MODE_AS: return FN<WPB, CAP, MODE_AS, true>(__VA_ARGS__);              \
+            case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, true>(__VA_ARGS__);    \
+            default: return hipErrorInvalidValue;                                       \
+        }                                                                               \
+    }                                                                                   \
+    switch (mode) {                                                                     \
+        case MODE_INFA: return FN<WPB, CAP, MODE_INFA, false>(__VA_ARGS__);             \
+        case MODE_TKO: return FN<WPB, CAP, MODE_TKO, false>(__VA_ARGS__);               \
+        case MODE_AS: return FN<WPB, CAP, MODE_AS, false>(__VA_ARGS__);                 \
+        case MODE_ADMATCH: return FN<WPB, CAP, MODE_ADMATCH, false>(__VA_ARGS__);       \
+        case MODE_HEUR: return FN<WPB, CAP, MODE_HEUR, false>(__VA_ARGS__);             \
+        case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, false>(__VA_ARGS__);       \
+        case MODE_LOG: return FN<WPB, CAP, MODE_LOG, false>(__VA_ARGS__);               \
+        default: return hipErrorInvalidValue;                                           \
     }
 template <int WPB, int CAP>
-hipError_t engine_launch_t(int mode, int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
+hipError_t engine_launch_t(int mode, bool hashed, int grid, size_t lds, hipStream_t st, const DevGraph& g,
+                           const ScanArgs& a) {
     ADM_DISPATCH_MODE(engine_launch_m, grid, lds, st, g, a)
 }
 template <int WPB, int CAP>
-hipError_t engine_occ_t(int mode, size_t lds, int* blocks) {
+hipError_t engine_occ_t(int mode, bool hashed, size_t lds, int* blocks) {
     ADM_DISPATCH_MODE(engine_occ_m, lds, blocks)
 }
 template <int BS>
@@ -922,13 +1307,13 @@ hipError_t plain_launch_t(int n, size_t lds, hipStream_t st, const DevGraph& g, 
         default: return hipErrorInvalidValue;                          \
     }
 
-hipError_t launch_engine(int wpb, int cap, int mode, int grid, size_t lds, hipStream_t st, const DevGraph& g,
-                         const ScanArgs& a) {
-    ADM_DISPATCH_ENGINE(engine_launch_t, mode, grid, lds, st, g, a)
+hipError_t launch_engine(int wpb, int cap, int mode, bool hashed, int grid, size_t lds, hipStream_t st,
+                         const DevGraph& g, const ScanArgs& a) {
+    ADM_DISPATCH_ENGINE(engine_launch_t, mode, hashed, grid, lds, st, g, a)
 }
 
-hipError_t engine_occupancy(int wpb, int cap, int mode, size_t lds, int* blocks) {
-    ADM_DISPATCH_ENGINE(engine_occ_t, mode, lds, blocks)
+hipError_t engine_occupancy(int wpb, int cap, int mode, bool hashed, size_t lds, int* blocks) {
+    ADM_DISPATCH_ENGINE(engine_occ_t, mode, hashed, lds, blocks)
 }
 
 hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {

@@ -71,8 +71,8 @@ def lib():
             "ita_scan_device": (i32, [vp, vp, vp, i32, i32, i32, vp, i32]),
             "ita_collect": (i32, [vp, i32, vp, vp, i64]),
             "ita_last_stats": (i32, [vp, vp, vp]),
-            "ita_set_variant": (i32, [vp, i32, i32, i32]),
-            "ita_get_variant": (i32, [vp, vp, vp, vp]),
+            "ita_set_variant": (i32, [vp, i32, i32, i32, i32]),
+            "ita_get_variant": (i32, [vp, vp, vp, vp, vp]),
             "ita_set_selector": (i32, [vp, vp, i32, i32]),
             "ita_set_log": (i32, [vp, i64, i32]),
             "ita_fetch_log": (i64, [vp, vp, i64]),
@@ -190,13 +190,15 @@ class Scratch:
         return ms.value, steps.tolist()
 
     # -- configuration ---------------------------------------------------
-    def set_variant(self, wpb: int = 2, cap: int = 512, plain_bs: int = 64):
-        _err(lib().ita_set_variant(self._h, wpb, cap, plain_bs))
+    def set_variant(self, wpb: int = 2, cap: int = 512, plain_bs: int = 64, store: int = 0):
+        """store 0: LDS bitmaps; 1: hashed active lists (AS and learned modes)."""
+        _err(lib().ita_set_variant(self._h, wpb, cap, plain_bs, store))
 
     def variant(self):
-        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _err(lib().ita_get_variant(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return a.value, b.value, c.value
+        """(wpb, cap, plain_bs, store)"""
+        v = [ctypes.c_int32() for _ in range(4)]
+        _err(lib().ita_get_variant(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def set_selector(self, weights: Optional[np.ndarray], every: int = 0):
         if weights is None:

@@ -121,7 +121,7 @@ def main(argv=None):
                   "best_method_counts": np.bincount(ds.best(), minlength=3).tolist()},
         "heldout_steps": {"n": ds_ev.n, "learned": rl, "heuristic": rh},
         "heldout_scan": {"strings": ev.inputs.n, "bytes": int(ev.inputs.offsets[-1]), **timings},
-        "variant": dict(zip(("wpb", "cap", "plain_bs"), s.variant())),
+        "variant": dict(zip(("wpb", "cap", "plain_bs", "store"), s.variant())),
     }
     with open(os.path.splitext(args.out)[0] + "_report.json", "w") as f:
         json.dump(report, f, indent=1)

@@ -107,15 +107,20 @@ int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids
 int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3);
 
 /* Engine launch variant: waves (strings) per workgroup in {1,2,4}, active-list
- * capacity in {256,512,1024}; plain_bs in {64,128,256} for the plain kernel. */
-int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs);
-int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs);
+ * capacity in {256,512,1024}; plain_bs in {64,128,256} for the plain kernel;
+ * store selects the AS and learned modes' state storage: 0 = two S-bit
+ * bitmaps per string in LDS, 1 = active lists with LDS hash tables (LDS per
+ * string independent of S) and the S-bit bitmaps in device memory, used by
+ * dense steps only.  The other modes always use LDS bitmaps. */
+int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs, int32_t store);
+int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs, int32_t* store);
 
 /* Learned step selector: 5,443 packed float weights (layout in
  * admatch_amd/csrc/device.hpp).  every == 0 (default): each wave caches the
  * network's choice per input byte, tagged with the active-count bucket and
- * list validity, and evaluates the network on a miss; every > 0: evaluate
- * every `every` steps.  weights == NULL removes the selector (the heuristic
+ * list validity; a miss takes the AS method (iNFA when the list overflowed)
+ * and queues the step's features, which the network answers after the
+ * string; every > 0: evaluate every `every` steps.  weights == NULL removes the selector (the heuristic
  * is used). */
 int ita_set_selector(ita_handle* h, const float* weights, int32_t n, int32_t every);
 

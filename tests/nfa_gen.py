@@ -5,6 +5,11 @@
 * shift_nfa: many transitions per (symbol, offset) for the offsets that
   stress the TKO word shifts (0, +-1, +-31, +-32, +-33, +-63, +-64, +-65), so
   each becomes a top-k group (size > S/16).
+* burst_nfa + burst_strings: an always-active start state whose rows reach
+  S/26 states per letter, most of which stay active while letters continue
+  and die at other bytes, so the active set swings between a few states and
+  more than a thousand within one string (list overflow, dense steps and the
+  return to the list form), plus word-boundary transitions.
 """
 from __future__ import annotations
 
@@ -67,6 +72,43 @@ def shift_nfa(seed: int, S: int = 600, alphabet: bytes = b"abc") -> str:
     for k, s in enumerate(rng.choice(S, size=30, replace=False)):
         lines.append(f"{s} : accepting {k + 1}")
     return "\n".join(lines) + "\n"
+
+
+def burst_nfa(seed: int, S: int = 3000, n_rules: int = 50) -> str:
+    rng = np.random.default_rng(seed)
+    lines = ["# burst", str(S), "0 : initial", "0 -> 0 : 0|255"]
+    for i in range(1, S):
+        lines.append(f"0 -> {i} : {97 + i % 26}")
+        if rng.random() < 0.7:
+            lines.append(f"{i} -> {i} : 97|122")
+    for _ in range(S // 2):  # letter chains between the burst states
+        a, b = (int(x) for x in rng.integers(1, S, 2))
+        lines.append(f"{a} -> {b} : {int(rng.integers(97, 123))}")
+    for _ in range(S // 50):  # digit and space transitions
+        a, b = (int(x) for x in rng.integers(1, S, 2))
+        lines.append(f"{a} -> {b} : {int(rng.choice([32, 48, 49, 57]))}")
+    for s_ in rng.choice(np.arange(1, S), size=S // 100, replace=False):
+        lines.append(f"{int(s_)} -> {int(s_)} : 0|255")
+    for _ in range(20):
+        a, b = (int(x) for x in rng.integers(0, S, 2))
+        lines.append(f"{a} -> {b} : 256")
+    for k, s_ in enumerate(rng.choice(np.arange(1, S), size=2 * n_rules, replace=False)):
+        lines.append(f"{int(s_)} : accepting {1 + k % n_rules}")
+    return "\n".join(lines) + "\n"
+
+
+def burst_strings(seed: int, n: int, max_len: int = 400):
+    """Mostly letters with occasional spaces, digits and punctuation."""
+    rng = np.random.default_rng(seed)
+    other = np.frombuffer(b" 019.,-_", dtype=np.uint8)
+    out = []
+    for _ in range(n):
+        L = int(rng.integers(0, max_len + 1))
+        letters = rng.integers(97, 123, L).astype(np.uint8)
+        mask = rng.random(L) < 0.08
+        letters[mask] = rng.choice(other, int(mask.sum()))
+        out.append(letters.tobytes())
+    return out
 
 
 def brute_force(nfa_text: str, data: bytes):

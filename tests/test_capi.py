@@ -42,11 +42,20 @@ def test_info_without_gpu(gold_dir):
     info = s.info()
     assert info["states"] == 1997 and info["words"] == 32
     assert info["tko_groups"] > 0 and info["as_entries"] == info["transitions"] - info["wb_transitions"]
-    assert s.variant() == (2, 512, 64)
+    assert s.variant() == (2, 512, 64, 0)
+    lds_bitmaps = s.info()["engine_lds_bytes"]
     s.set_variant(4, 1024, 128)
-    assert s.variant() == (4, 1024, 128)
+    assert s.variant() == (4, 1024, 128, 0)
+    s.set_variant(4, 256, 128, store=1)
+    assert s.variant() == (4, 256, 128, 1)
+    # hashed storage: two lists, two tables of 2*cap slots and the miss queue per string
+    assert s.info()["engine_lds_bytes"] == 4 * (3 * 256 + 32) * 8
+    s.set_variant(2, 512, 64)
+    assert s.info()["engine_lds_bytes"] == lds_bitmaps
     with pytest.raises(E.AdmatchError):
         s.set_variant(3, 512, 64)
+    with pytest.raises(E.AdmatchError):
+        s.set_variant(2, 512, 64, store=2)
     s.close()
 
 

@@ -1,0 +1,104 @@
+"""GPU parity of the engine's two state storages (LDS bitmaps, and hashed
+active lists with HBM bitmaps) on hand-shaped rule sets that reach every
+storage transition: word-boundary chains, TKO shift groups, and active sets
+that swing between a few states and more than a thousand in one string
+(list overflow in an AS step, dense steps, the return to the list form).
+Every launch variant and selector mode against the CPU oracle, bit-exact."""
+import os
+
+import numpy as np
+import pytest
+
+from nfa_gen import burst_nfa, burst_strings, random_nfa, shift_nfa
+
+pytestmark = pytest.mark.gpu
+
+from admatch_amd import engine as E  # noqa: E402
+from admatch_amd import selector  # noqa: E402
+
+import oracle  # noqa: E402  (checker only)
+
+ALL_MODES = [E.MODE_INFA, E.MODE_TKO, E.MODE_AS, E.MODE_ADMATCH, E.MODE_HEURISTIC, E.MODE_LEARNED]
+HASHED_MODES = [E.MODE_AS, E.MODE_LEARNED]
+VARIANTS = [(2, 512, 0), (1, 256, 1), (2, 512, 1), (4, 256, 1), (1, 1024, 1), (4, 1024, 1)]
+
+
+def make_case(tmp_path, kind, seed):
+    rng = np.random.default_rng(seed)
+    if kind == "random":
+        text = random_nfa(seed, S=200, n_edges=900)
+        strs = [bytes(rng.choice(list(b"ab_ Z\x00"), size=int(rng.integers(0, 300)))) for _ in range(150)]
+    elif kind == "shift":
+        text = shift_nfa(seed)
+        strs = [bytes(rng.choice(list(b"abcx"), size=int(rng.integers(0, 300)))) for _ in range(150)]
+    else:
+        text = burst_nfa(seed)
+        strs = burst_strings(seed, 150)
+    path = str(tmp_path / f"{kind}{seed}.nfa")
+    with open(path, "w") as f:
+        f.write(text)
+    o = oracle.OracleNFA(path)
+    return path, strs, [o.scan(s) for s in strs]
+
+
+def first_diff(got, exp):
+    for i, (g, e) in enumerate(zip(got, exp)):
+        if g != e:
+            return i, g, e
+    return None
+
+
+@pytest.mark.parametrize("kind,seed", [("random", 0), ("random", 1), ("shift", 0), ("burst", 0), ("burst", 1)])
+def test_storages_vs_oracle(tmp_path, kind, seed):
+    path, strs, exp = make_case(tmp_path, kind, seed)
+    buf, off = E.pack_strings(strs)
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+    weights = np.load(selector.WEIGHTS_PATH) if os.path.exists(selector.WEIGHTS_PATH) else None
+    for wpb, cap, store in VARIANTS:
+        s.set_variant(wpb, cap, 64, store)
+        for mode in (HASHED_MODES if store else ALL_MODES):
+            for w in ([None, weights] if (mode == E.MODE_LEARNED and weights is not None) else [None]):
+                s.set_selector(w)
+                print(f"[case] {kind}{seed} wpb={wpb} cap={cap} store={store} mode={E.MODE_NAMES[mode]} "
+                      f"trained={w is not None}", flush=True)
+                s.scan_device(d_buf.ptr, d_off.ptr, len(strs), E.KIND_ENGINE, mode)
+                got = E.unpack(*s.collect(len(strs)))
+                assert got == exp, (wpb, cap, store, mode, first_diff(got, exp))
+    s.close()
+
+
+def test_hashed_graph_replay_and_growth(tmp_path):
+    # captured launches keep their HBM bitmaps valid when a larger batch
+    # grows the buffer (the graphs are dropped and captured again)
+    path, strs, exp = make_case(tmp_path, "burst", 2)
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    s.set_variant(2, 256, 64, 1)
+    for n in (8, 150, 8, 150):
+        buf, off = E.pack_strings(strs[:n])
+        d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+        for rep in range(2):
+            s.scan_device(d_buf.ptr, d_off.ptr, n, E.KIND_ENGINE, E.MODE_AS, use_graph=True)
+            got = E.unpack(*s.collect(n))
+            assert got == exp[:n], (n, rep, first_diff(got, exp[:n]))
+    s.close()
+
+
+@pytest.mark.parametrize("cfg,n", [("small", 1024), ("medium", 2048), ("large", 384)])
+def test_hashed_synthetic_configs(workloads, cfg, n):
+    w, path = workloads(cfg, n)
+    buf, off = w.inputs.buf, w.inputs.offsets
+    exp = oracle.OracleNFA(path).batch_scan(buf, off, nthreads=os.cpu_count() or 4)
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+    weights = np.load(selector.WEIGHTS_PATH) if os.path.exists(selector.WEIGHTS_PATH) else None
+    for cap in (256, 512):
+        s.set_variant(4, cap, 64, 1)
+        for mode, wt in ((E.MODE_AS, None), (E.MODE_LEARNED, None), (E.MODE_LEARNED, weights)):
+            s.set_selector(wt)
+            print(f"[case] {cfg} n={n} cap={cap} hashed mode={E.MODE_NAMES[mode]} trained={wt is not None}",
+                  flush=True)
+            s.scan_device(d_buf.ptr, d_off.ptr, off.size - 1, E.KIND_ENGINE, mode)
+            got = E.unpack(*s.collect(off.size - 1))
+            assert got == exp, (cap, mode, first_diff(got, exp))
+    s.close()

@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from conftest import golden_inputs, golden_json, golden_nfa
-from nfa_gen import brute_force, random_nfa, shift_nfa
+from nfa_gen import brute_force, burst_nfa, burst_strings, random_nfa, shift_nfa
 
 import oracle  # checker under test
 
@@ -63,6 +63,15 @@ def test_brute_force_shift_nfas(tmp_path, seed):
     rng = np.random.default_rng(seed)
     for _ in range(10):
         data = bytes(rng.choice(list(b"abcx"), size=int(rng.integers(0, 60))))
+        assert o.scan(data) == brute_force(text, data)
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_brute_force_burst_nfas(tmp_path, seed):
+    # active sets of up to ~2,000 states that collapse at non-letters
+    text = burst_nfa(seed, S=1500)
+    o = oracle.OracleNFA(write(tmp_path, "b.nfa", text))
+    for data in burst_strings(seed, 6, max_len=200):
         assert o.scan(data) == brute_force(text, data)
 
 
