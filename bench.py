@@ -115,6 +115,10 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests per arrival rate for the serving measurement (0: skip)")
     ap.add_argument("--serve-rates", default="50000,200000,800000")
+    ap.add_argument("--variant", default=None,
+                    help="wpb,cap,store: engine launch variant (default: the autotuned one)")
+    ap.add_argument("--tune", default=None, metavar="PATH",
+                    help="autotune the launch variant on this workload first and store it in PATH")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -142,12 +146,20 @@ def main():
         with open(tune_path) as f:
             tuned = json.load(f).get(str(info["hash"]))
     if tuned:
-        s.set_variant(tuned["wpb"], tuned["cap"], tuned.get("plain_bs", 64))
+        s.set_variant(tuned["wpb"], tuned["cap"], tuned.get("plain_bs", 64), tuned.get("store", 0))
     learned_ok = False
     if os.path.exists(SELECTOR_PATH):
         s.set_selector(np.load(SELECTOR_PATH), every=0)
         learned_ok = True
     d_buf, d_off = E.DeviceArray(w.inputs.buf), E.DeviceArray(w.inputs.offsets)
+    if args.tune:
+        from admatch_amd import autotune
+        best = autotune.tune(s, d_buf, d_off, n, mode=E.MODE_AS, reps=1, log=log)
+        autotune.store(info["hash"], best, path=args.tune)
+    if args.variant:
+        wpb, cap, store = (int(x) for x in args.variant.split(","))
+        s.set_variant(wpb, cap, s.variant()[2], store)
+    log(f"variant (wpb, cap, plain_bs, store) = {s.variant()}")
 
     modes = {
         "plain": (E.KIND_PLAIN, E.MODE_INFA),
@@ -203,7 +215,7 @@ def main():
         "data": "synthetic (seeded rules and HTTP-like strings)",
         "config": {"workload": f"{args.config}: {w.ruleset.state_count} states from "
                                f"{len(w.ruleset.rules)} rules, {n} strings, {total_bytes} B",
-                   "variant": dict(zip(("wpb", "cap", "plain_bs"), s.variant())),
+                   "variant": dict(zip(("wpb", "cap", "plain_bs", "store"), s.variant())),
                    "learned_selector": learned_ok},
         "modes": {k: {"gbs": round(v["gbs"], 4), "ms_per_step": round(v["ms_per_step"], 4),
                       "kernel_ms": round(v["kernel_ms"], 4), "method_steps": v["method_steps"]}
