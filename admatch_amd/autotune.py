@@ -2,7 +2,7 @@
 
 Sweeps the engine's state storage {LDS bitmaps, hashed lists with HBM
 bitmaps}, waves per workgroup {1, 2, 4} and active-list capacity
-{256, 512, 1024}, and the plain iNFA kernel's workgroup size {64, 128, 256},
+{128, 256, 512, 1024}, and the plain iNFA kernel's workgroup size {64, 128, 256},
 on a device-resident sample of the workload; keeps the fastest by hipEvent
 kernel time (all variants must give identical results).  Results are cached
 in admatch_amd/data/autotune.json keyed by the rule set's content hash.
@@ -39,7 +39,7 @@ def tune(s: E.Scratch, d_buf, d_off, n: int, mode: int = E.MODE_HEURISTIC, reps:
     stores = (0, 1) if mode in (E.MODE_AS, E.MODE_LEARNED) else (0,)
     for store in stores:
         for wpb in (1, 2, 4):
-            for cap in (256, 512, 1024):
+            for cap in (128, 256, 512, 1024):
                 try:
                     s.set_variant(wpb, cap, 64, store)
                     ms = _time(s, d_buf, d_off, n, E.KIND_ENGINE, mode, reps)

@@ -15,7 +15,8 @@
 //    The active set is kept both as a bitmap and, while it has at most LCAP
 //    states, as a list, so the AS and iNFA methods never touch all W words.
 //  * bin kernels: longest-first processing order by length bins.
-//  * MLP selector inference is fused into the engine's step loop.
+//  * learned selector: a per-wave decision cache consulted every step; the
+//    MLP answers cache misses after each string (answer_pending).
 #include "device.hpp"
 
 namespace admatch {
@@ -56,15 +57,16 @@ struct HashSet {
     __device__ __forceinline__ static uint32_t slot(uint32_t d) {
         return (d * 2654435761u) >> (32 - __builtin_ctz(uint32_t(H)));
     }
-    // true if d was not in the set
+    // true if d was not in the set.  One compare-and-swap on the home slot;
+    // the probe loop runs only after a collision with another key.
     __device__ __forceinline__ bool insert(uint32_t d) const {
         uint32_t i = slot(d);
-        for (;;) {
-            const uint32_t old = atomicCAS(&tab[i], kEmpty, d);
-            if (old == kEmpty) return true;
-            if (old == d) return false;
+        uint32_t old = atomicCAS(&tab[i], kEmpty, d);
+        while (old != kEmpty && old != d) {
             i = (i + 1) & uint32_t(H - 1);
+            old = atomicCAS(&tab[i], kEmpty, d);
         }
+        return old == kEmpty;
     }
     __device__ __forceinline__ bool contains(uint32_t d) const {
         uint32_t i = slot(d);
@@ -77,8 +79,10 @@ struct HashSet {
     }
     // every slot free again (whole wave)
     __device__ __forceinline__ void clear() const {
+        static_assert(H % 256 == 0, "the wave clears 256 slots per store");
         uint4* t4 = reinterpret_cast<uint4*>(tab);
-        for (int j = lane_id(); j < H / 4; j += 64) t4[j] = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
+#pragma unroll
+        for (int r = 0; r < H / 256; ++r) t4[r * 64 + lane_id()] = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
     }
 };
 
@@ -416,34 +420,40 @@ __device__ __forceinline__ int heuristic_choice(const DevGraph& g, const StepCtx
     return int64_t(s.T) <= tko_cost ? M_INFA : M_TKO;
 }
 
-__device__ __forceinline__ void make_features(const DevGraph& g, const StepCtx& s, float* f) {
-    f[0] = __log2f(1.f + float(s.n_act));
-    f[1] = 100.f * float(s.n_act) / float(g.S);
-    f[2] = __log2f(1.f + float(s.T));
-    f[3] = float(s.K);
-    f[4] = __log2f(1.f + float(s.Lft));
-    f[5] = __log2f(float(g.W));
-    f[6] = word_char(uint32_t(s.c)) ? 1.f : 0.f;
-    f[7] = float(s.c >> 5);
-    f[8] = float(s.pos) / float(s.len > 0 ? s.len : 1);
-    f[9] = __log2f(1.f + float(s.len));
-    f[10] = float(s.sparse);
-    f[11] = s.last_m == M_INFA ? 1.f : 0.f;
-    f[12] = s.last_m == M_TKO ? 1.f : 0.f;
-    f[13] = s.last_m == M_AS ? 1.f : 0.f;
-    f[14] = __log2f(1.f + float(s.len - s.pos));
-    f[15] = __log2f(1.f + float(s.K) * float(g.W));
+// Selector feature i (0..15) of a step; the wave holds feature l in lane l.
+__device__ __forceinline__ float feature(const DevGraph& g, const StepCtx& s, int i) {
+    switch (i) {
+        case 0: return __log2f(1.f + float(s.n_act));
+        case 1: return 100.f * float(s.n_act) / float(g.S);
+        case 2: return __log2f(1.f + float(s.T));
+        case 3: return float(s.K);
+        case 4: return __log2f(1.f + float(s.Lft));
+        case 5: return __log2f(float(g.W));
+        case 6: return word_char(uint32_t(s.c)) ? 1.f : 0.f;
+        case 7: return float(s.c >> 5);
+        case 8: return float(s.pos) / float(s.len > 0 ? s.len : 1);
+        case 9: return __log2f(1.f + float(s.len));
+        case 10: return float(s.sparse);
+        case 11: return s.last_m == M_INFA ? 1.f : 0.f;
+        case 12: return s.last_m == M_TKO ? 1.f : 0.f;
+        case 13: return s.last_m == M_AS ? 1.f : 0.f;
+        case 14: return __log2f(1.f + float(s.len - s.pos));
+        case 15: return __log2f(1.f + float(s.K) * float(g.W));
+        default: return 0.f;
+    }
 }
 
-// 16 -> 64 -> 64 -> 3 MLP evaluated by one wave: lane j is hidden unit j.
-// W1 is stored input-major [16][64], W2 as [64 in][64 out], so every
-// weight load is one coalesced 256-byte row.  Returns argmin of the three
-// predicted log step costs among the allowed methods.
-__device__ int mlp_choice(const float* w, const float* f, bool allow_as) {
+// 16 -> 64 -> 64 -> 3 MLP evaluated by one wave: lane j is hidden unit j;
+// fl holds feature l in lane l.  W1 is stored input-major [16][64], W2 as
+// [64 in][64 out], so every weight load is one coalesced 256-byte row.
+// Returns argmin of the three predicted log step costs among the allowed
+// methods.
+__device__ int mlp_choice(const float* w, float fl, bool allow_as) {
     const int j = lane_id();
     float h = w[kMlpB1 + j];
-#pragma unroll
-    for (int i = 0; i < kFeatures; ++i) h += w[kMlpW1 + i * kHidden + j] * f[i];
+#pragma unroll 4
+    for (int i = 0; i < kFeatures; ++i)
+        h += w[kMlpW1 + i * kHidden + j] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl), i));
     h = fmaxf(h, 0.f);
     float h2 = w[kMlpB2 + j];
 #pragma unroll 8
@@ -465,7 +475,6 @@ __device__ int mlp_choice(const float* w, const float* f, bool allow_as) {
     }
     return arg;
 }
-
 
 // Per-wave state of the learned selector.  dcache holds one decision per
 // byte value: byte c's entry is byte (c & 3) of lane c >> 2, tagged with the
@@ -515,9 +524,7 @@ __device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& 
                 load_stats();
                 ctx.n_act = total_active();
                 if (sel.cached < 0 || (b % a.mlp_every) == 0) {
-                    float f[kFeatures];
-                    make_features(g, ctx, f);
-                    sel.cached = mlp_choice(a.mlp, f, true);
+                    sel.cached = mlp_choice(a.mlp, feature(g, ctx, lane_id()), true);
                 }
                 m = sel.cached;
                 if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
@@ -571,9 +578,7 @@ __device__ void answer_pending(const DevGraph& g, const ScanArgs& a, Selector& s
         qc.T = st.x;
         qc.K = st.y;
         qc.Lft = st.z;
-        float f[kFeatures];
-        make_features(g, qc, f);
-        const int mm = mlp_choice(a.mlp, f, qc.sparse != 0);
+        const int mm = mlp_choice(a.mlp, feature(g, qc, lane_id()), qc.sparse != 0);
         const int sh = (qc.c & 3) * 8;
         if (lane_id() == (qc.c >> 2) && int((sel.dcache >> sh) & 0xffu) >> 2 == q[1])
             sel.dcache = (sel.dcache & ~(0xffu << sh)) | (uint32_t((q[1] << 2) | mm) << sh);
@@ -791,15 +796,14 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                     wave_sync();
                 }
             }
-            float f[kFeatures];
-            make_features(g, ctx, f);
+            const float fl = feature(g, ctx, lane_id());
             unsigned long long slot = 0;
             if (lane_id() == 0) slot = atomicAdd(a.log_top, 1ull);
             slot = __shfl(static_cast<long long>(slot), 0, 64);
             if (int64_t(slot) < a.log_cap) {
                 float* rec = a.log_buf + int64_t(slot) * kLogRecord;
                 const int l = lane_id();
-                if (l < kFeatures) rec[l] = f[l];
+                if (l < kFeatures) rec[l] = fl;
                 if (l < 3) rec[kFeatures + l] = tm[l];
                 if (l == 0) rec[kFeatures + 3] = float(m);
             }
@@ -829,7 +833,9 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             n_act = uniform(rebuild_list<CAP>(cur, W, cl));
             sparse = n_act <= CAP;
         }
-        msteps[m] += 1;
+        msteps[0] += int(m == M_INFA);
+        msteps[1] += int(m == M_TKO);
+        msteps[2] += int(m == M_AS);
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
@@ -1065,7 +1071,9 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             imp = -1;
         }
         ctx.c_prev = c;
-        msteps[m] += 1;
+        msteps[0] += int(m == M_INFA);
+        msteps[1] += int(m == M_TKO);
+        msteps[2] += int(m == M_AS);
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
@@ -1116,13 +1124,27 @@ struct LaneH {
     static constexpr size_t words() { return size_t(3) * CAP + kPending * 4; }
 };
 
+// Build knob: ADM_HASHED_WAVES > 0 asks the compiler to fit that many waves
+// per SIMD in registers (it may spill to scratch to do so).
+#ifndef ADM_HASHED_WAVES
+#define ADM_HASHED_WAVES 0
+#endif
+#if ADM_HASHED_WAVES > 0
+#define ADM_HASHED_ATTR __attribute__((amdgpu_waves_per_eu(ADM_HASHED_WAVES)))
+#else
+#define ADM_HASHED_ATTR
+#endif
+
+// The rule set is read through a.graph (device memory) rather than passed
+// by value: its rarely used fields are then loaded where they are used
+// instead of occupying scalar registers for the whole kernel.
 template <int WPB, int CAP, int MODE>
-__global__ __launch_bounds__(64 * WPB) void engine_h_kernel(DevGraph g, ScanArgs a) {
+__global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(ScanArgs a) {
     extern __shared__ uint64_t lds[];
-    __shared__ int4 s_stats[256];
-    for (int i = threadIdx.x; i < 256; i += 64 * WPB) s_stats[i] = g.sym_stats[i];
-    __syncthreads();
-    g.sym_stats = s_stats;
+    // constant address space: loads of the rule set's fields may be reordered
+    // and repeated freely (nothing in the kernel writes them)
+    using ConstGraph = const __attribute__((address_space(4))) DevGraph;
+    const DevGraph& g = *(const DevGraph*)(ConstGraph*)(a.graph);
     const int wave = threadIdx.x >> 6;
     uint64_t* base = lds + LaneH<CAP>::words() * wave;
     uint32_t* listA = reinterpret_cast<uint32_t*>(base);
@@ -1226,30 +1248,32 @@ namespace {
 constexpr int kEngineDynLds = 160 * 1024 - int(sizeof(int4)) * 256;
 
 template <int WPB, int CAP, int MODE, bool HASHED>
-constexpr auto engine_fn() {
-    if constexpr (HASHED) return &engine_h_kernel<WPB, CAP, MODE>;
-    else return &engine_kernel<WPB, CAP, MODE>;
+const void* engine_fn() {
+    if constexpr (HASHED) return reinterpret_cast<const void*>(&engine_h_kernel<WPB, CAP, MODE>);
+    else return reinterpret_cast<const void*>(&engine_kernel<WPB, CAP, MODE>);
 }
 template <int WPB, int CAP, int MODE, bool HASHED>
 hipError_t engine_launch_m(int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
-    constexpr auto fn = engine_fn<WPB, CAP, MODE, HASHED>();
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute(engine_fn<WPB, CAP, MODE, HASHED>(), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   kEngineDynLds);
         (void)hipGetLastError();  // an unsupported attribute must not fail the launch check
         attr = true;
     }
-    hipLaunchKernelGGL(fn, dim3(grid), dim3(64 * WPB), lds, st, g, a);
+    if constexpr (HASHED) {
+        hipLaunchKernelGGL((engine_h_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, a);
+    } else {
+        hipLaunchKernelGGL((engine_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
+    }
     return hipGetLastError();
 }
 template <int WPB, int CAP, int MODE, bool HASHED>
 hipError_t engine_occ_m(size_t lds, int* blocks) {
-    constexpr auto fn = engine_fn<WPB, CAP, MODE, HASHED>();
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(fn), hipFuncAttributeMaxDynamicSharedMemorySize,
+    (void)hipFuncSetAttribute(engine_fn<WPB, CAP, MODE, HASHED>(), hipFuncAttributeMaxDynamicSharedMemorySize,
                               kEngineDynLds);
     (void)hipGetLastError();
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, fn, 64 * WPB, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, engine_fn<WPB, CAP, MODE, HASHED>(), 64 * WPB, lds);
 }
 // hashed storage is built for the modes that run mostly AS steps
 #define ADM_DISPATCH_MODE(FN, ...)                                                      \
@@ -1295,6 +1319,9 @@ hipError_t plain_launch_t(int n, size_t lds, hipStream_t st, const DevGraph& g, 
 
 #define ADM_DISPATCH_ENGINE(FN, ...)                                   \
     switch (wpb * 10000 + cap) {                                       \
+        case 10128: return FN<1, 128>(__VA_ARGS__);                    \
+        case 20128: return FN<2, 128>(__VA_ARGS__);                    \
+        case 40128: return FN<4, 128>(__VA_ARGS__);                    \
         case 10256: return FN<1, 256>(__VA_ARGS__);                    \
         case 20256: return FN<2, 256>(__VA_ARGS__);                    \
         case 40256: return FN<4, 256>(__VA_ARGS__);                    \
