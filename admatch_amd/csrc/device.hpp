@@ -55,7 +55,6 @@ struct ScanArgs {
     int32_t log_stride;       // sample one step in log_stride
     unsigned long long* method_steps;  // [4] steps per method, or null
     uint64_t* dense;          // hashed storage: two W-word bitmaps per wave of the grid, all zero
-    const DevGraph* graph;    // hashed storage: the rule set in device memory
 };
 
 enum SelectMode : int32_t {

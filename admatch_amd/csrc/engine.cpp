@@ -197,7 +197,6 @@ void Engine::release() {
     for (Context* c : contexts_) c->release();
     for (void* p : dev_allocs_) (void)hipFree(p);
     dev_allocs_.clear();
-    d_graph_ = nullptr;
     table_bytes_ = 0;
     dfree(d_mlp_);
     dfree(d_log_);
@@ -283,8 +282,6 @@ int Engine::upload(std::string& err) {
 #undef ADM_PUT
     dev_.wb_begin = g_.sym_off[kWordBoundary];
     dev_.wb_count = g_.wb_count();
-    // the table of device pointers itself, for kernels that read it from memory
-    if (int rc = put(&dev_, sizeof(dev_), reinterpret_cast<const void**>(&d_graph_))) return rc;
     dev_ready_ = true;
     return ADM_OK;
 }
@@ -310,7 +307,6 @@ ScanArgs Engine::make_args(Context& c, const uint8_t* d_text, const int64_t* d_o
     a.log_top = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrLogTop);
     a.log_stride = log_stride_ > 0 ? log_stride_ : 1;
     a.method_steps = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrMethods);
-    a.graph = d_graph_;
     return a;
 }
 

@@ -157,7 +157,6 @@ class Engine {
     Graph g_;
     bool loaded_ = false, dev_ready_ = false;
     DevGraph dev_{};
-    const DevGraph* d_graph_ = nullptr;  // device copy of dev_ (in dev_allocs_)
     std::vector<void*> dev_allocs_;
     size_t table_bytes_ = 0;
     int cu_count_ = 0;

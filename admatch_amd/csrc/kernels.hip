@@ -1135,16 +1135,13 @@ struct LaneH {
 #define ADM_HASHED_ATTR
 #endif
 
-// The rule set is read through a.graph (device memory) rather than passed
-// by value: its rarely used fields are then loaded where they are used
-// instead of occupying scalar registers for the whole kernel.
 template <int WPB, int CAP, int MODE>
-__global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(ScanArgs a) {
+__global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevGraph g, ScanArgs a) {
     extern __shared__ uint64_t lds[];
-    // constant address space: loads of the rule set's fields may be reordered
-    // and repeated freely (nothing in the kernel writes them)
-    using ConstGraph = const __attribute__((address_space(4))) DevGraph;
-    const DevGraph& g = *(const DevGraph*)(ConstGraph*)(a.graph);
+    __shared__ int4 s_stats[256];
+    for (int i = threadIdx.x; i < 256; i += 64 * WPB) s_stats[i] = g.sym_stats[i];
+    __syncthreads();
+    g.sym_stats = s_stats;
     const int wave = threadIdx.x >> 6;
     uint64_t* base = lds + LaneH<CAP>::words() * wave;
     uint32_t* listA = reinterpret_cast<uint32_t*>(base);
@@ -1262,7 +1259,7 @@ hipError_t engine_launch_m(int grid, size_t lds, hipStream_t st, const DevGraph&
         attr = true;
     }
     if constexpr (HASHED) {
-        hipLaunchKernelGGL((engine_h_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, a);
+        hipLaunchKernelGGL((engine_h_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
     } else {
         hipLaunchKernelGGL((engine_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
     }

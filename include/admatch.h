@@ -107,7 +107,7 @@ int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids
 int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3);
 
 /* Engine launch variant: waves (strings) per workgroup in {1,2,4}, active-list
- * capacity in {256,512,1024}; plain_bs in {64,128,256} for the plain kernel;
+ * capacity in {128,256,512,1024}; plain_bs in {64,128,256} for the plain kernel;
  * store selects the AS and learned modes' state storage: 0 = two S-bit
  * bitmaps per string in LDS, 1 = active lists with LDS hash tables (LDS per
  * string independent of S) and the S-bit bitmaps in device memory, used by
