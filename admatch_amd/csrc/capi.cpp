@@ -281,6 +281,12 @@ int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap) {
         case 13: return give(g.alw_dst.data(), int64_t(g.alw_dst.size()), 4);
         case 14: return give(g.pair_off.data(), int64_t(g.pair_off.size()), 4);
         case 15: return give(g.pair_dst.data(), int64_t(g.pair_dst.size()), 4);
+        case 16: {
+            std::vector<uint32_t> t(admatch::kPolicyTableWords);
+            std::string err;
+            if (int rc = h->eng.policy_table(t.data(), err)) return fail(rc, err);
+            return give(t.data(), int64_t(t.size()), 4);
+        }
         default: return fail(ITA_ERR_ARG, "bad table id");
     }
 }

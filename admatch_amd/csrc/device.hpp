@@ -55,6 +55,7 @@ struct ScanArgs {
     int32_t log_stride;       // sample one step in log_stride
     unsigned long long* method_steps;  // [4] steps per method, or null
     uint64_t* dense;          // hashed storage: two W-word bitmaps per wave of the grid, all zero
+    const uint32_t* policy;   // learned selector table (2 bits per entry) or null
 };
 
 enum SelectMode : int32_t {
@@ -65,11 +66,13 @@ enum SelectMode : int32_t {
     MODE_HEUR = 4,     // AdMatch rule extended with an iNFA cost term
     MODE_LEARNED = 5,  // MLP selector, MODE_HEUR as fallback
     MODE_LOG = 6,      // run all three methods per step and log timings
+    MODE_LEARNED_EVERY = 7,  // internal: the network evaluated in the scan every mlp_every steps
 };
 
 enum Method : int32_t { M_INFA = 0, M_TKO = 1, M_AS = 2 };
 
 constexpr int kFeatures = 16;
+constexpr int kPolicyTableWords = 512;  // 16 buckets x 2 list states x 256 bytes x 2 bits
 constexpr int kHidden = 64;
 constexpr int kOutputs = 3;
 // Packed MLP layout (floats): W1[64][16], b1[64], W2[64][64], b2[64],
@@ -81,7 +84,8 @@ constexpr int kMlpB2 = kMlpW2 + kHidden * kHidden;
 constexpr int kMlpW3 = kMlpB2 + kHidden;
 constexpr int kMlpB3 = kMlpW3 + kOutputs * kHidden;
 constexpr int kMlpParams = kMlpB3 + kOutputs;  // 5,443
-constexpr int kLogRecord = kFeatures + kOutputs + 1;  // features, 3 times, chosen
+// log record: features, 3 cycle counts, chosen method, byte, explicit active count
+constexpr int kLogRecord = kFeatures + kOutputs + 3;
 
 __device__ __forceinline__ bool word_char(uint32_t c) {
     return (c - 'A' < 26u) || (c - 'a' < 26u) || (c - '0' < 10u) || c == '_';

@@ -173,10 +173,10 @@ int Engine::load(const char* path, std::string& err) {
 }
 
 size_t Engine::engine_lds_bytes(const Variant& v, int32_t mode) const {
-    // per wave: lists (cap u32 each) and the 8-record miss queue, plus either
-    // two W-word bitmaps or two hash tables of 2*cap u32
-    if (hashed(v, mode)) return size_t(v.wpb) * (size_t(3) * v.cap + 32) * 8;
-    return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap) + 32) * 8;
+    // per wave: two lists of cap u32, plus either two W-word bitmaps or two
+    // hash tables of 2*cap u32
+    if (hashed(v, mode)) return size_t(v.wpb) * (size_t(3) * v.cap) * 8;
+    return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap)) * 8;
 }
 
 size_t Engine::plain_lds_bytes() const { return size_t(3) * g_.W * 8 + 258 * 4; }
@@ -199,6 +199,7 @@ void Engine::release() {
     dev_allocs_.clear();
     table_bytes_ = 0;
     dfree(d_mlp_);
+    dfree(d_policy_);
     dfree(d_log_);
     log_cap_ = 0;
     dev_ready_ = false;
@@ -302,6 +303,7 @@ ScanArgs Engine::make_args(Context& c, const uint8_t* d_text, const int64_t* d_o
     a.overflow = c.d_ctr + kCtrOverflow;
     a.mlp = d_mlp_;
     a.mlp_every = mlp_every_ >= 0 ? mlp_every_ : 0;
+    a.policy = (d_mlp_ && a.mlp_every == 0) ? d_policy_ : nullptr;
     a.log_buf = d_log_;
     a.log_cap = log_cap_;
     a.log_top = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrLogTop);
@@ -374,6 +376,12 @@ int Engine::scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off,
     }
     if (int rc = c.init(err)) return rc;
     if (int rc = c.ensure_results(n, err)) return rc;
+    if (kind == KIND_ENGINE && mode == MODE_LEARNED) {
+        // the learned kernel reads the policy table; without weights the
+        // heuristic decides, with every > 0 the network runs in the scan
+        if (!d_mlp_) mode = MODE_HEUR;
+        else if (mlp_every_ > 0) mode = MODE_LEARNED_EVERY;
+    }
     if (kind == KIND_ENGINE && n > 0 && hashed(var_, mode)) {
         int grid = 0;
         if (int rc = engine_grid(mode, n, &grid, err)) return rc;
@@ -528,6 +536,7 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
     drop_all_graphs();
     if (w == nullptr) {
         dfree(d_mlp_);
+        dfree(d_policy_);
         return ADM_OK;
     }
     if (n != kMlpParams) {
@@ -537,6 +546,19 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
     if (!d_mlp_) ADM_HIP(hipMalloc(&d_mlp_, sizeof(float) * kMlpParams));
     ADM_HIP(hipMemcpy(d_mlp_, w, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
     mlp_every_ = every >= 0 ? every : 0;
+    // the network's decision per (active-count bucket, list validity, byte)
+    if (!d_policy_) ADM_HIP(hipMalloc(&d_policy_, sizeof(uint32_t) * kPolicyTableWords));
+    ADM_HIP(launch_policy(dev_, d_mlp_, d_policy_, nullptr));
+    ADM_HIP(hipStreamSynchronize(nullptr));
+    return ADM_OK;
+}
+
+int Engine::policy_table(uint32_t* out, std::string& err) {
+    if (!d_policy_) {
+        err = "no learned selector set";
+        return ADM_ERR_STATE;
+    }
+    ADM_HIP(hipMemcpy(out, d_policy_, sizeof(uint32_t) * kPolicyTableWords, hipMemcpyDeviceToHost));
     return ADM_OK;
 }
 

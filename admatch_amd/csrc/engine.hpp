@@ -135,6 +135,7 @@ class Engine {
     void set_variant(const Variant& v);
     const Variant& variant() const { return var_; }
     int set_mlp(const float* w, int32_t n, int32_t every, std::string& err);
+    int policy_table(uint32_t* out, std::string& err);  // kPolicyTableWords u32
     bool has_mlp() const { return d_mlp_ != nullptr; }
     // step logging for the learned selector (MODE_LOG)
     int set_log(int64_t cap_records, int32_t stride, std::string& err);
@@ -163,7 +164,8 @@ class Engine {
     size_t max_lds_ = 0;
     Variant var_;
     float* d_mlp_ = nullptr;
-    int32_t mlp_every_ = 0;  // 0: per-wave decision cache
+    uint32_t* d_policy_ = nullptr;  // learned selector table (kPolicyTableWords)
+    int32_t mlp_every_ = 0;  // 0: policy table
     float* d_log_ = nullptr;
     int64_t log_cap_ = 0;
     int32_t log_stride_ = 64;

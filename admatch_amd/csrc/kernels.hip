@@ -15,8 +15,9 @@
 //    The active set is kept both as a bitmap and, while it has at most LCAP
 //    states, as a list, so the AS and iNFA methods never touch all W words.
 //  * bin kernels: longest-first processing order by length bins.
-//  * learned selector: a per-wave decision cache consulted every step; the
-//    MLP answers cache misses after each string (answer_pending).
+//  * learned selector: the MLP is evaluated once per (byte, active-count
+//    bucket, list validity) into a 2-bit policy table (policy_kernel); a
+//    step reads its method with one scalar load.
 #include "device.hpp"
 
 namespace admatch {
@@ -476,28 +477,56 @@ __device__ int mlp_choice(const float* w, float fl, bool allow_as) {
     return arg;
 }
 
-// Per-wave state of the learned selector.  dcache holds one decision per
-// byte value: byte c's entry is byte (c & 3) of lane c >> 2, tagged with the
-// bucket of the listed active count and list validity.  pend queues the
-// features of up to kPending cache misses; the network answers them after
-// the string (answer_pending), outside the step loop.
-constexpr int kPending = 8;
+// Learned policy table: entry (bucket, sparse, c) is the method the network
+// picks for a step on byte c with an explicit active count n in bucket
+// min(15, floor(log2(n + 1))) and list validity `sparse`; 2 bits per entry,
+// 16 entries per u32, [kPolicyBuckets][2][16] words.
+constexpr int kPolicyBuckets = 16;
+constexpr int kPolicyWords = kPolicyBuckets * 2 * 16;
+static_assert(kPolicyWords == kPolicyTableWords, "policy table size");
+// Step context at which the network is evaluated for a table entry: the
+// bucket's middle count plus the mean always-active row, a step in the
+// middle of a kPolicyLen-byte string, and the method a step of that list
+// validity usually follows.
+constexpr int kPolicyLen = 1024;
 
-struct Selector {
-    uint32_t dcache;
-    int* pend;  // kPending records of 8 ints in LDS
-    int npend;
-    int cached;  // mlp_every > 0: the last evaluated decision
-};
+__device__ __forceinline__ int policy_bucket(int n_act) { return min(kPolicyBuckets - 1, 31 - __clz(n_act + 1)); }
+
+__device__ __forceinline__ StepCtx policy_ctx(const DevGraph& g, int c, int bucket, bool sparse) {
+    StepCtx s;
+    s.c = c;
+    s.c_prev = -1;
+    s.n_act = (bucket == 0 ? 0 : (3 << (bucket - 1)) - 1) + g.alw_total / 256;
+    s.sparse = sparse;
+    const int4 st = g.sym_stats[c];
+    s.T = st.x;
+    s.K = st.y;
+    s.Lft = st.z;
+    s.last_m = sparse ? M_AS : M_INFA;
+    s.pos = kPolicyLen / 2;
+    s.len = kPolicyLen;
+    return s;
+}
+
+// One wave per table entry; table zeroed beforehand.
+__global__ __launch_bounds__(64) void policy_kernel(DevGraph g, const float* mlp, uint32_t* table) {
+    const int e = blockIdx.x;  // (bucket * 2 + sparse) * 256 + c
+    const int c = e & 255;
+    const int sp = (e >> 8) & 1;
+    const int bucket = e >> 9;
+    const StepCtx s = policy_ctx(g, c, bucket, sp != 0);
+    const int m = mlp_choice(mlp, feature(g, s, lane_id()), sp != 0);
+    if (lane_id() == 0) atomicOr(&table[(e >> 4)], uint32_t(m) << ((c & 15) * 2));
+}
 
 // Method for the step on ctx.c.  total_active() counts the active states
-// including the implicitly held ones (selector features).
+// including the implicitly held ones (selector features).  cached: the last
+// network decision of the every-k-steps mode.
 template <int MODE, typename TOTAL>
 __device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& a, int sid, StepCtx& ctx, int n_act,
-                                             bool sparse, const TOTAL& total_active, Selector& sel, bool& log_step) {
+                                             bool sparse, const TOTAL& total_active, int& cached, bool& log_step) {
     const int c = ctx.c;
     const int b = ctx.pos;
-    ctx.n_act = MODE == MODE_LEARNED ? n_act : total_active();
     // per-byte statistics, read only by the selectors that use them
     auto load_stats = [&]() {
         const int4 st = g.sym_stats[c];
@@ -512,87 +541,46 @@ __device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& 
         case MODE_AS: m = sparse ? M_AS : M_INFA; break;
         case MODE_ADMATCH:
             load_stats();
+            ctx.n_act = total_active();
             m = (sparse && ctx.K > ctx.n_act) ? M_AS : M_TKO;
             break;
-        case MODE_LEARNED:
-            if (a.mlp == nullptr) {
-                load_stats();
-                ctx.n_act = total_active();
-                m = heuristic_choice(g, ctx);
-            } else if (a.mlp_every > 0) {
-                // re-evaluate every mlp_every steps
-                load_stats();
-                ctx.n_act = total_active();
-                if (sel.cached < 0 || (b % a.mlp_every) == 0) {
-                    sel.cached = mlp_choice(a.mlp, feature(g, ctx, lane_id()), true);
-                }
-                m = sel.cached;
-                if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
-            } else {
-                // decision cache; a miss takes the active-state method when
-                // the list is valid (iNFA otherwise) and queues the features
-                const int tag = ((31 - __clz(n_act + 1)) << 1) | int(sparse);
-                const int sh = (c & 3) * 8;
-                const int e = int((uint32_t(__builtin_amdgcn_readlane(int(sel.dcache), c >> 2)) >> sh) & 0xffu);
-                if ((e >> 2) == tag) {
-                    m = e & 3;
-                } else {
-                    m = sparse ? M_AS : M_INFA;
-                    if (lane_id() == (c >> 2))
-                        sel.dcache = (sel.dcache & ~(0xffu << sh)) | (uint32_t((tag << 2) | m) << sh);
-                    if (sel.npend < kPending) {
-                        const int l = lane_id();
-                        const int val = l == 0 ? c : l == 1 ? tag : l == 2 ? total_active() : l == 3 ? int(sparse)
-                                      : l == 4 ? b : l == 5 ? ctx.len : ctx.last_m;
-                        if (l < 7) sel.pend[sel.npend * 8 + l] = val;
-                        ++sel.npend;
-                    }
-                }
-            }
+        case MODE_LEARNED: {
+            // policy table (set whenever this mode is launched): one scalar load
+            using CU32 = const __attribute__((address_space(4))) uint32_t;
+            const CU32* pt = (CU32*)(a.policy);
+            const uint32_t wv = pt[(policy_bucket(n_act) * 2 + int(sparse)) * 16 + (c >> 4)];
+            m = int((wv >> ((c & 15) * 2)) & 3u);
+            break;
+        }
+        case MODE_LEARNED_EVERY:
+            // the network itself, re-evaluated every mlp_every steps
+            load_stats();
+            ctx.n_act = total_active();
+            if (cached < 0 || (b % a.mlp_every) == 0) cached = mlp_choice(a.mlp, feature(g, ctx, lane_id()), true);
+            m = cached;
+            if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
             break;
         case MODE_LOG:
             load_stats();
+            ctx.n_act = total_active();
             m = heuristic_choice(g, ctx);
             log_step = ((uint64_t(sid) * 2654435761ull + uint64_t(b)) % uint64_t(a.log_stride)) == 0;
             break;
         default:
             load_stats();
+            ctx.n_act = total_active();
             m = heuristic_choice(g, ctx);
             break;
     }
     return m;
 }
 
-// Answers the queued decision-cache misses with the network (whole wave).
-__device__ void answer_pending(const DevGraph& g, const ScanArgs& a, Selector& sel) {
-    for (int k = 0; k < sel.npend; ++k) {
-        const int* q = sel.pend + k * 8;
-        StepCtx qc;
-        qc.c = q[0];
-        qc.n_act = q[2];
-        qc.sparse = q[3];
-        qc.pos = q[4];
-        qc.len = q[5];
-        qc.last_m = q[6];
-        const int4 st = g.sym_stats[qc.c];
-        qc.T = st.x;
-        qc.K = st.y;
-        qc.Lft = st.z;
-        const int mm = mlp_choice(a.mlp, feature(g, qc, lane_id()), qc.sparse != 0);
-        const int sh = (qc.c & 3) * 8;
-        if (lane_id() == (qc.c >> 2) && int((sel.dcache >> sh) & 0xffu) >> 2 == q[1])
-            sel.dcache = (sel.dcache & ~(0xffu << sh)) | (uint32_t((q[1] << 2) | mm) << sh);
-    }
-    sel.npend = 0;
-    wave_sync();
-}
-
 // ------------------------------------------------------------ engine
 
 template <int CAP>
 struct Lane {
-    // u64 words per wave: two bitmaps, two lists, the miss queue
-    static constexpr size_t words(int W) { return size_t(2) * W + CAP + kPending * 4; }
+    // u64 words per wave: two bitmaps, two lists
+    static constexpr size_t words(int W) { return size_t(2) * W + CAP; }
 };
 
 // Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
@@ -636,7 +624,7 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
 
 template <int CAP, int MODE>
 __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
-                            uint32_t* listA, uint32_t* listB, Selector& sel, int* msteps) {
+                            uint32_t* listA, uint32_t* listB, int* msteps) {
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int L = int(a.offsets[sid + 1] - beg);
@@ -728,7 +716,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     ctx.len = L;
     ctx.last_m = M_INFA;
     ctx.c_prev = -1;
-    sel.cached = -1;
+    int cached = -1;
     for (int b = 0; b < L; ++b) {
         const int c = int(win.at(b));
         RowSlice pair_next{0u, 0, 0};
@@ -749,7 +737,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         ctx.sparse = sparse;
         ctx.pos = b;
         bool log_step = false;
-        int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, sel, log_step);
+        int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
         if (m == M_AS && !(sparse && can_as)) m = M_INFA;
         bool fsp;
         int n_fut;
@@ -806,6 +794,8 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 if (l < kFeatures) rec[l] = fl;
                 if (l < 3) rec[kFeatures + l] = tm[l];
                 if (l == 0) rec[kFeatures + 3] = float(m);
+                if (l == 1) rec[kFeatures + 4] = float(c);
+                if (l == 2) rec[kFeatures + 5] = float(n_act);
             }
             m = M_INFA;
         }
@@ -851,7 +841,6 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     wave_sync();
     clear_bits<CAP>(cur, cl, n_act, sparse, W);
     wave_sync();
-    if (MODE == MODE_LEARNED && a.mlp != nullptr) answer_pending(g, a, sel);
 }
 
 // ------------------------------------------------------------ hashed storage
@@ -893,8 +882,7 @@ __device__ void dense_method(const DevGraph& g, int m, int c, const uint64_t* cu
 
 template <int CAP, int MODE>
 __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* gA, uint64_t* gB,
-                              uint32_t* hA, uint32_t* hB, uint32_t* listA, uint32_t* listB, Selector& sel,
-                              int* msteps) {
+                              uint32_t* hA, uint32_t* hB, uint32_t* listA, uint32_t* listB, int* msteps) {
     using HS = HashSet<2 * CAP>;
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
@@ -1001,7 +989,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     ctx.len = L;
     ctx.last_m = M_INFA;
     ctx.c_prev = -1;
-    sel.cached = -1;
+    int cached = -1;
     for (int b = 0; b < L; ++b) {
         const int c = int(win.at(b));
         RowSlice pair_next{0u, 0, 0};
@@ -1021,7 +1009,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         ctx.sparse = sparse;
         ctx.pos = b;
         bool log_step = false;
-        int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, sel, log_step);
+        int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
         if (m == M_AS && !(sparse && can_as)) m = M_INFA;
         int n_fut = 0;
         bool fsp = false;
@@ -1115,13 +1103,12 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         dense_sync();
     }
     wave_sync();
-    if (MODE == MODE_LEARNED && a.mlp != nullptr) answer_pending(g, a, sel);
 }
 
 template <int CAP>
 struct LaneH {
-    // u64 words per wave: two lists of CAP u32, two tables of 2*CAP u32, the miss queue
-    static constexpr size_t words() { return size_t(3) * CAP + kPending * 4; }
+    // u64 words per wave: two lists of CAP u32, two tables of 2*CAP u32
+    static constexpr size_t words() { return size_t(3) * CAP; }
 };
 
 // Build knob: ADM_HASHED_WAVES > 0 asks the compiler to fit that many waves
@@ -1148,7 +1135,6 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
     uint32_t* listB = listA + CAP;
     uint32_t* hA = listB + CAP;
     uint32_t* hB = hA + 2 * CAP;
-    Selector sel{~0u, reinterpret_cast<int*>(hB + 2 * CAP), 0, -1};
     uint64_t* gA = a.dense + (size_t(blockIdx.x) * WPB + wave) * 2 * size_t(g.W);
     uint64_t* gB = gA + g.W;
     HashSet<2 * CAP>{hA}.clear();
@@ -1161,7 +1147,7 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
         if (idx >= a.n) break;
         const int sid = a.order ? a.order[idx] : idx;
-        scan_string_h<CAP, MODE>(g, a, sid, gA, gB, hA, hB, listA, listB, sel, msteps);
+        scan_string_h<CAP, MODE>(g, a, sid, gA, gB, hA, hB, listA, listB, msteps);
     }
     if (a.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)
@@ -1183,7 +1169,6 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
     uint64_t* bitsB = base + g.W;
     uint32_t* listA = reinterpret_cast<uint32_t*>(base + 2 * g.W);
     uint32_t* listB = listA + CAP;
-    Selector sel{~0u, reinterpret_cast<int*>(base + 2 * g.W + CAP), 0, -1};  // no cached decisions
     for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
     wave_sync();
     int msteps[3] = {0, 0, 0};  // steps per method, this wave
@@ -1193,7 +1178,7 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
         if (idx >= a.n) break;
         const int sid = a.order ? a.order[idx] : idx;
-        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, sel, msteps);
+        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, msteps);
     }
     if (a.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)
@@ -1289,6 +1274,7 @@ hipError_t engine_occ_m(size_t lds, int* blocks) {
         case MODE_HEUR: return FN<WPB, CAP, MODE_HEUR, false>(__VA_ARGS__);             \
         case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, false>(__VA_ARGS__);       \
         case MODE_LOG: return FN<WPB, CAP, MODE_LOG, false>(__VA_ARGS__);               \
+        case MODE_LEARNED_EVERY: return FN<WPB, CAP, MODE_LEARNED_EVERY, false>(__VA_ARGS__); \
         default: return hipErrorInvalidValue;                                           \
     }
 template <int WPB, int CAP>
@@ -1347,6 +1333,13 @@ hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGrap
         case 256: return plain_launch_t<256>(n, lds, st, g, a);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_policy(const DevGraph& g, const float* mlp, uint32_t* table, hipStream_t st) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<int32_t*>(table), 0,
+                       int64_t(kPolicyWords));
+    hipLaunchKernelGGL(policy_kernel, dim3(kPolicyBuckets * 2 * 256), dim3(64), 0, st, g, mlp, table);
+    return hipGetLastError();
 }
 
 hipError_t launch_fill(int32_t* p, int32_t v, int64_t n, hipStream_t st) {

@@ -36,7 +36,7 @@ MODE_NAMES = {MODE_INFA: "infa", MODE_TKO: "tko", MODE_AS: "as", MODE_ADMATCH: "
 
 ITA_OK, ITA_ERR_CAPACITY = 0, -5
 MLP_PARAMS = 5443
-LOG_RECORD = 20
+LOG_RECORD = 22
 
 EXPORTED = [
     "ita_create", "ita_alloc", "ita_free", "ita_destroy", "ita_scan", "ita_batched_scan", "ita_scan_device",
@@ -207,6 +207,10 @@ class Scratch:
         w = np.ascontiguousarray(weights, dtype=np.float32)
         _err(lib().ita_set_selector(self._h, _p(w), w.size, every))
 
+    def policy_table(self) -> np.ndarray:
+        """The learned selector's table (512 u32) of the current weights."""
+        return self.table(16)
+
     def set_log(self, cap_records: int, stride: int):
         _err(lib().ita_set_log(self._h, cap_records, stride))
 
@@ -226,7 +230,7 @@ class Scratch:
 
     def table(self, which: int) -> np.ndarray:
         dt = {4: np.uint64, 9: np.uint64, 1: np.uint32, 6: np.uint32, 7: np.uint32, 8: np.uint32, 10: np.uint32,
-              11: np.uint32, 13: np.uint32, 14: np.uint32, 15: np.uint32}.get(which, np.int32)
+              11: np.uint32, 13: np.uint32, 14: np.uint32, 15: np.uint32, 16: np.uint32}.get(which, np.int32)
         n = lib().ita_table(self._h, which, None, 0)
         if n < 0:
             _err(int(n))

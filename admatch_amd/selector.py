@@ -5,10 +5,11 @@ same state for one sampled step in ``stride`` and records the step's 16
 features and each method's cycle count.  A 16 -> 64 -> 64 -> 3 MLP is then
 trained on the GPU (ita_selector_train: the project's own forward/backward
 and Adam kernels) to predict log2 cycles per method; at scan time the
-engine evaluates it inside the step loop and runs the method with the
-lowest prediction (the heuristic decides when the network picks the active-
-state method for a state held only as a bitmap).  Results do not depend on
-the choice: every method computes the same next state set.
+network is evaluated once per (active-count bucket, list validity, byte)
+into a 2-bit policy table (the engine's policy_kernel) that each step reads
+with one scalar load; the method with the lowest predicted cost runs.
+Results do not depend on the choice: every method computes the same next
+state set.
 """
 from __future__ import annotations
 
@@ -27,7 +28,7 @@ WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", 
 
 FEATURE_NAMES = [
     "log2(1+active)", "active %", "log2(1+|T_c|)", "k_c", "log2(1+leftover_c)", "log2(W)", "word(c)",
-    "c/32", "pos/len", "log2(1+len)", "list valid", "last=iNFA", "last=TKO", "last=AS", "log2(1+last cycles)",
+    "c/32", "pos/len", "log2(1+len)", "list valid", "last=iNFA", "last=TKO", "last=AS", "log2(1+remaining)",
     "log2(1+k_c*W)",
 ]
 
@@ -53,6 +54,8 @@ class Dataset:
     x: np.ndarray     # [n,16] float32 features
     y: np.ndarray     # [n,3] log2 cycles (0 where not timed)
     mask: np.ndarray  # [n,3] 1 where timed
+    c: Optional[np.ndarray] = None       # [n] byte of the step
+    n_expl: Optional[np.ndarray] = None  # [n] explicit (listed) active count
 
     @property
     def n(self):
@@ -64,12 +67,15 @@ class Dataset:
 
 
 def from_log(records: np.ndarray) -> Dataset:
-    """Log records (20 floats: 16 features, 3 cycle counts or -1, chosen)."""
+    """Log records (22 floats: 16 features, 3 cycle counts or -1, chosen,
+    byte, explicit active count)."""
     x = records[:, :F].astype(np.float32)
     t = records[:, F:F + O].astype(np.float64)
     mask = (t > 0).astype(np.float32)
     y = np.where(t > 0, np.log2(np.maximum(t, 1.0)), 0.0).astype(np.float32)
-    return Dataset(x=x, y=y, mask=mask)
+    c = records[:, F + 4].astype(np.int64) if records.shape[1] > F + 4 else None
+    ne = records[:, F + 5].astype(np.int64) if records.shape[1] > F + 5 else None
+    return Dataset(x=x, y=y, mask=mask, c=c, n_expl=ne)
 
 
 def collect(scratch: E.Scratch, d_buf, d_off, n: int, cap: int = 1 << 20, stride: int = 16) -> np.ndarray:
@@ -164,6 +170,55 @@ def heuristic_choice(ds: Dataset, W: int) -> np.ndarray:
     out = np.where(T <= K * W + Lft, 0, 1)
     out[sparse & (K > active)] = 2
     return out
+
+
+# ---------------------------------------------------------------- policy table
+# Mirrors policy_kernel (admatch_amd/csrc/kernels.hip): entry (bucket b, list
+# valid v, byte c) holds the method the network picks at a representative
+# step context; bits 2*(c%16) of word (2*b + v)*16 + c//16.
+POLICY_BUCKETS = 16
+POLICY_LEN = 1024
+
+
+def policy_bucket(n_expl) -> np.ndarray:
+    n = np.asarray(n_expl, dtype=np.int64)
+    return np.minimum(POLICY_BUCKETS - 1, np.floor(np.log2(n + 1)).astype(np.int64))
+
+
+def policy_inputs(scratch: E.Scratch):
+    """Features [16*2*256, 16] (float64) of every table entry, and whether
+    the entry may pick AS (list valid)."""
+    info = scratch.info()
+    W, S = info["words"], info["states"]
+    soff = scratch.table(0).astype(np.int64)
+    koff = scratch.table(2).astype(np.int64)
+    loff = scratch.table(5).astype(np.int64)
+    alw_total = int(scratch.table(12)[256])
+    T, K, Lft = np.diff(soff)[:256], np.diff(koff)[:256], np.diff(loff)[:256]
+    rows, allow = [], []
+    for b in range(POLICY_BUCKETS):
+        n = (0 if b == 0 else (3 << (b - 1)) - 1) + alw_total // 256
+        for v in (0, 1):
+            for c in range(256):
+                word = chr(c).isascii() and (chr(c).isalnum() or c == ord("_"))
+                rows.append([np.log2(1 + n), 100.0 * n / S, np.log2(1 + T[c]), K[c], np.log2(1 + Lft[c]), np.log2(W),
+                             float(word), float(c >> 5), 0.5, np.log2(1 + POLICY_LEN), float(v), float(v == 0), 0.0,
+                             float(v == 1), np.log2(1 + POLICY_LEN // 2), np.log2(1 + K[c] * W)])
+                allow.append(v == 1)
+    return np.array(rows, dtype=np.float64), np.array(allow)
+
+
+def pack_policy(choice: np.ndarray) -> np.ndarray:
+    """Methods in entry order -> the u32[512] table."""
+    t = np.zeros(POLICY_BUCKETS * 2 * 16, dtype=np.uint32)
+    e = np.arange(choice.size)
+    np.bitwise_or.at(t, e >> 4, (choice.astype(np.uint32) & 3) << ((e & 15) * 2).astype(np.uint32))
+    return t
+
+
+def policy_lookup(table: np.ndarray, n_expl, sparse, c) -> np.ndarray:
+    e = (policy_bucket(n_expl) * 2 + np.asarray(sparse, dtype=np.int64)) * 256 + np.asarray(c, dtype=np.int64)
+    return ((table[e >> 4] >> ((e & 15) * 2).astype(np.uint32)) & 3).astype(np.int64)
 
 
 def load_weights(path: Optional[str] = None) -> Optional[np.ndarray]:

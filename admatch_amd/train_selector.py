@@ -7,7 +7,8 @@
    all three step methods on one sampled step in `stride`;
 2. training: Adam on the GPU (ita_selector_train), log2-cycle targets;
 3. evaluation on held-out strings of the large workload (seed 103):
-   offline regret of the network vs the heuristic on held-out logs, and
+   offline regret of the network, of the policy table the engine reads,
+   and of the heuristic on held-out logs, and
    measured scan time of the learned selector vs the heuristic selectors,
    with bit-identical results required;
 4. writes the weights (.npy) and a JSON report beside them.
@@ -93,10 +94,12 @@ def main(argv=None):
     pred = selector.gpu_forward(ds_ev.x, w)
     rl = selector.regret(ds_ev, selector.choose(pred, allow))
     rh = selector.regret(ds_ev, selector.heuristic_choice(ds_ev, W))
-    log(f"held-out steps: learned {rl}, heuristic {rh}")
+    # what the engine reads: the network folded into the policy table
+    s.set_selector(w, every=0)
+    rt = selector.regret(ds_ev, selector.policy_lookup(s.policy_table(), ds_ev.n_expl, allow, ds_ev.c))
+    log(f"held-out steps: network {rl}, policy table {rt}, heuristic {rh}")
 
     # measured scans on held-out strings
-    s.set_selector(w, every=0)
     timings, ref = {}, None
     for name, mode in (("heuristic", E.MODE_HEURISTIC), ("admatch", E.MODE_ADMATCH), ("learned", E.MODE_LEARNED),
                        ("as", E.MODE_AS)):
@@ -119,7 +122,7 @@ def main(argv=None):
         "train": {"strings": tr.inputs.n, "logged_steps": ds.n, "stride": args.stride, "epochs": args.epochs,
                   "batch": args.batch, "lr": args.lr, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
                   "best_method_counts": np.bincount(ds.best(), minlength=3).tolist()},
-        "heldout_steps": {"n": ds_ev.n, "learned": rl, "heuristic": rh},
+        "heldout_steps": {"n": ds_ev.n, "network": rl, "policy_table": rt, "heuristic": rh},
         "heldout_scan": {"strings": ev.inputs.n, "bytes": int(ev.inputs.offsets[-1]), **timings},
         "variant": dict(zip(("wpb", "cap", "plain_bs", "store"), s.variant())),
     }

@@ -116,16 +116,17 @@ int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs, i
 int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs, int32_t* store);
 
 /* Learned step selector: 5,443 packed float weights (layout in
- * admatch_amd/csrc/device.hpp).  every == 0 (default): each wave caches the
- * network's choice per input byte, tagged with the active-count bucket and
- * list validity; a miss takes the AS method (iNFA when the list overflowed)
- * and queues the step's features, which the network answers after the
- * string; every > 0: evaluate every `every` steps.  weights == NULL removes the selector (the heuristic
+ * admatch_amd/csrc/device.hpp).  every == 0 (default): the network is
+ * evaluated on the GPU once per (active-count bucket, list validity, byte)
+ * into a policy table (table 16 of ita_table) that every step reads;
+ * every > 0: evaluate the network in the scan every `every` steps.  weights == NULL removes the selector (the heuristic
  * is used). */
 int ita_set_selector(ita_handle* h, const float* weights, int32_t n, int32_t every);
 
 /* Step logging for selector training (ITA_MODE_LOG): room for cap records
- * of 20 floats, one step sampled in every `stride`. */
+ * of 22 floats (16 features, cycles of iNFA, TKO and AS or -1, the
+ * heuristic's method, the byte, the explicit active count), one step
+ * sampled in every `stride`. */
 int ita_set_log(ita_handle* h, int64_t cap_records, int32_t stride);
 int64_t ita_fetch_log(ita_handle* h, float* out, int64_t cap_records);
 
@@ -143,8 +144,12 @@ int ita_info(ita_handle* h, int64_t* out11);
  * (uint32), 9 init/persistent/accept vectors (3*W u64), 10 device AS
  * table (256*S uint32), 11 its multi-row store (uint32), 12 always-active
  * row offsets (257 int32), 13 always-active rows (uint32), 14 two-byte
- * row offsets (65537 uint32), 15 two-byte rows (uint32).  Returns the
- * element count; copies min(count, cap) elements when out != NULL. */
+ * row offsets (65537 uint32), 15 two-byte rows (uint32), 16 the learned
+ * selector's policy table (512 uint32, after ita_set_selector: method of
+ * (bucket b, list valid v, byte c) in bits 2*(c%16).. of word
+ * (2*b + v)*16 + c/16, bucket = min(15, floor(log2(explicit count + 1)))).
+ * Returns the element count; copies min(count, cap) elements when
+ * out != NULL. */
 int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap);
 
 /* Learned-selector training on the GPU (no counterpart in the reference):

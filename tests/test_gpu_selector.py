@@ -64,6 +64,31 @@ def test_training_reduces_loss():
     assert np.mean(pred.argmin(1) == y.argmin(1)) > 0.9
 
 
+@pytest.mark.parametrize("seed", [3, 4])
+def test_policy_table_matches_network(workloads, seed):
+    # random weights give varied decisions; the table must hold, for every
+    # (bucket, list validity, byte), the FP64 network's choice at the entry's
+    # step context (a near tie may go either way in FP32)
+    w, path = workloads("medium", 64)
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    wts = selector.init_weights(seed)
+    wts[-3:] = np.array([0.0, 0.1, -0.1], dtype=np.float32)
+    s.set_selector(wts)
+    e = np.arange(selector.POLICY_BUCKETS * 2 * 256)
+    got = selector.policy_lookup(s.policy_table(), np.where(e >> 9 == 0, 0, (3 << np.maximum((e >> 9) - 1, 0)) - 1),
+                                 (e >> 8) & 1, e & 255)
+    X, allow = selector.policy_inputs(s)
+    pred = mlp_ref.forward(X, wts)
+    pred[~allow, 2] = np.inf
+    exp = pred.argmin(1)
+    assert set(np.unique(exp).tolist()) == {0, 1, 2}
+    bad = np.nonzero(got != exp)[0]
+    gap = np.abs(pred[bad, got[bad]] - pred[bad, exp[bad]])
+    assert (gap < 1e-3).all(), (bad[:10], gap[:10])
+    assert bad.size <= 0.01 * e.size
+    s.close()
+
+
 def test_log_mode_and_learned_gain(workloads):
     w, path = workloads("large", 1024)
     buf, off = w.inputs.buf, w.inputs.offsets
@@ -74,7 +99,7 @@ def test_log_mode_and_learned_gain(workloads):
     ref = s.collect(n)
     # logging runs all three methods per sampled step and keeps results
     rec = selector.collect(s, d_buf, d_off, n, cap=1 << 18, stride=8)
-    assert rec.shape[0] > 1000 and rec.shape[1] == 20
+    assert rec.shape[0] > 1000 and rec.shape[1] == 22
     s.set_log(1 << 18, 8)
     s.scan_device(d_buf.ptr, d_off.ptr, n, E.KIND_ENGINE, E.MODE_LOG)
     got = s.collect(n)
@@ -82,9 +107,12 @@ def test_log_mode_and_learned_gain(workloads):
     s.set_log(0, 8)
     ds = selector.from_log(rec)
     assert (ds.mask[:, 0] == 1).all() and (ds.mask[:, 1] == 1).all()
+    assert ((ds.c >= 0) & (ds.c < 256)).all() and (ds.n_expl >= 0).all()
+    # the logged total count includes the implicitly held states
+    assert (np.rint(np.exp2(ds.x[:, 0]) - 1) >= ds.n_expl).all()
     half = ds.n // 2
     tr = selector.Dataset(ds.x[:half], ds.y[:half], ds.mask[:half])
-    te = selector.Dataset(ds.x[half:], ds.y[half:], ds.mask[half:])
+    te = selector.Dataset(ds.x[half:], ds.y[half:], ds.mask[half:], ds.c[half:], ds.n_expl[half:])
     wts, hist = selector.train(tr, epochs=20, batch=512, lr=2e-3, seed=0)
     assert hist[-1] < hist[0]
     pred = selector.gpu_forward(te.x, wts)
@@ -92,8 +120,12 @@ def test_log_mode_and_learned_gain(workloads):
     heur = selector.regret(te, selector.heuristic_choice(te, s.info()["words"]))
     # the network's choices cost fewer measured cycles than the heuristic's
     assert learned["picked_cycles"] < heur["picked_cycles"], (learned, heur)
-    # and the learned engine returns the same rule ids
+    # so do the policy table's, which the engine actually reads
     s.set_selector(wts)
+    table = selector.policy_lookup(s.policy_table(), te.n_expl, te.x[:, 10] > 0, te.c)
+    tab = selector.regret(te, table)
+    assert tab["picked_cycles"] < heur["picked_cycles"], (tab, heur)
+    # and the learned engine returns the same rule ids
     for every in (0, 4):
         s.set_selector(wts, every=every)
         s.scan_device(d_buf.ptr, d_off.ptr, n, E.KIND_ENGINE, E.MODE_LEARNED)
