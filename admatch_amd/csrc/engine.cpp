@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <cstring>
 
 #include "launch.hpp"
@@ -347,6 +348,24 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     return ADM_OK;
 }
 
+int32_t Engine::kernel_mode(int32_t mode) const {
+    // the learned kernel reads the policy table; without weights the
+    // heuristic decides, with every > 0 the network runs in the scan
+    if (mode != MODE_LEARNED) return mode;
+    if (!d_mlp_) return MODE_HEUR;
+    return mlp_every_ > 0 ? MODE_LEARNED_EVERY : MODE_LEARNED;
+}
+
+int Engine::prepare_dense(Context& c, int32_t mode, std::string& err) {
+    mode = kernel_mode(mode);
+    if (!hashed(var_, mode)) return ADM_OK;
+    // sized for the largest grid of this variant, so that batches of any size
+    // reuse it
+    int grid = 0;
+    if (int rc = engine_grid(mode, std::numeric_limits<int32_t>::max(), &grid, err)) return rc;
+    return c.ensure_dense(size_t(grid) * var_.wpb * 2 * size_t(g_.W), err);
+}
+
 int Engine::engine_grid(int32_t mode, int32_t n, int* grid, std::string& err) {
     const size_t lds = engine_lds_bytes(var_, mode);
     if (lds + sizeof(int4) * 256 > max_lds_) {
@@ -359,8 +378,8 @@ int Engine::engine_grid(int32_t mode, int32_t n, int* grid, std::string& err) {
         err = "engine variant does not fit on a CU";
         return ADM_ERR_TOO_LARGE;
     }
-    const int need = (n + var_.wpb - 1) / var_.wpb;
-    *grid = std::max(1, std::min(blocks * cu_count_, need));
+    const int64_t need = (int64_t(n) + var_.wpb - 1) / var_.wpb;
+    *grid = int(std::max<int64_t>(1, std::min<int64_t>(int64_t(blocks) * cu_count_, need)));
     return ADM_OK;
 }
 
@@ -376,17 +395,9 @@ int Engine::scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off,
     }
     if (int rc = c.init(err)) return rc;
     if (int rc = c.ensure_results(n, err)) return rc;
-    if (kind == KIND_ENGINE && mode == MODE_LEARNED) {
-        // the learned kernel reads the policy table; without weights the
-        // heuristic decides, with every > 0 the network runs in the scan
-        if (!d_mlp_) mode = MODE_HEUR;
-        else if (mlp_every_ > 0) mode = MODE_LEARNED_EVERY;
-    }
-    if (kind == KIND_ENGINE && n > 0 && hashed(var_, mode)) {
-        int grid = 0;
-        if (int rc = engine_grid(mode, n, &grid, err)) return rc;
-        if (int rc = c.ensure_dense(size_t(grid) * var_.wpb * 2 * size_t(g_.W), err)) return rc;
-    }
+    if (kind == KIND_ENGINE) mode = kernel_mode(mode);
+    if (kind == KIND_ENGINE && n > 0)
+        if (int rc = prepare_dense(c, mode, err)) return rc;
     hipStream_t st = c.stream;
     ADM_HIP(hipEventRecord(c.ev0, st));
     if (use_graph && n > 0) {

@@ -135,6 +135,10 @@ class Engine {
     void set_variant(const Variant& v);
     const Variant& variant() const { return var_; }
     int set_mlp(const float* w, int32_t n, int32_t every, std::string& err);
+    // HBM bitmaps of the hashed storage for context c (no-op for LDS storage)
+    int prepare_dense(Context& c, int32_t mode, std::string& err);
+    // the engine kernel a public mode runs (the learned mode's depends on the selector)
+    int32_t kernel_mode(int32_t mode) const;
     int policy_table(uint32_t* out, std::string& err);  // kPolicyTableWords u32
     bool has_mlp() const { return d_mlp_ != nullptr; }
     // step logging for the learned selector (MODE_LOG)

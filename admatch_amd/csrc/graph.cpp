@@ -356,7 +356,7 @@ void build_as(Graph& g) {
                 row.insert(std::lower_bound(row.begin(), row.end(), uint32_t(s)), uint32_t(s));
             // always-active destinations are held implicitly by the AS step
             row.erase(std::remove_if(row.begin(), row.end(), always), row.end());
-            uint32_t& ent = g.as_tab[size_t(c) * g.S + s];
+            uint32_t& ent = g.as_tab[size_t(s) * 256 + c];
             if (row.empty()) continue;
             if (row.size() == 1) {
                 ent = (1u << 24) | row[0];

@@ -58,8 +58,9 @@ struct Graph {
     // mask instead).
     std::vector<uint32_t> as_off, as_dst;
 
-    // Device form of the AS rows, one 32-bit entry per (byte, state), byte
-    // major (as_tab[c*S + s]): count in the top 8 bits (0 = empty, 255 =
+    // Device form of the AS rows, one 32-bit entry per (state, byte), state
+    // major (as_tab[s*256 + c], so the 256 entries of an active state share
+    // one kilobyte of cache lines): count in the top 8 bits (0 = empty, 255 =
     // long row whose count is as_multi[idx]), then either the single
     // destination (count 1) or the row's index into as_multi.  Rows of
     // persistent states include the state itself, so the AS step needs no

@@ -87,27 +87,28 @@ struct HashSet {
     }
 };
 
-// Byte i of a string, read through a 256-byte window held one u32 per lane.
+// Byte i of a string, read through a 256-byte window held one u32 per lane
+// (strings shorter than 2^31 bytes).
 struct ByteWindow {
     const uint8_t* p;
-    int64_t len;
-    int64_t base;  // first byte index of the window
-    uint32_t w;    // lane l holds bytes base+4l .. base+4l+3
+    int len;
+    int base;    // first byte index of the window
+    uint32_t w;  // lane l holds bytes base+4l .. base+4l+3
 
-    __device__ void load(int64_t b) {
+    __device__ void load(int b) {
         base = b;
-        const int64_t i = b + 4 * lane_id();
+        const int i = b + 4 * lane_id();
         uint32_t v = 0;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (i + k < len) v |= uint32_t(p[i + k]) << (8 * k);
         w = v;
     }
-    __device__ uint32_t at(int64_t i) {  // i uniform, base <= i
-        if (i - base >= 256) load(i);
-        const int off = int(i - base);
+    __device__ int at(int i) {  // i uniform, base <= i
+        if (uint32_t(i - base) >= 256u) load(i);
+        const int off = i - base;
         const uint32_t v = uint32_t(__builtin_amdgcn_readlane(int(w), off >> 2));
-        return (v >> (8 * (off & 3))) & 0xffu;
+        return int((v >> (8 * (off & 3))) & 0xffu);
     }
 };
 
@@ -353,11 +354,12 @@ struct RowSlice {
 template <int CAP, typename SET>
 __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_pair, const uint32_t* cl, int n_cur,
                        const SET& fut, uint32_t* fl) {
-    const uint32_t* tab = g.as_tab + size_t(c) * g.S;
+    // state-major table: entry (s, c) at s*256 + c
+    const uint32_t* tab = g.as_tab + c;
     // the first chunk's table gathers are in flight while the two-byte rows
     // are merged
     const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
-    const uint32_t e0 = (sv0 & kTag) ? 0u : tab[sv0 & kIdMask];
+    const uint32_t e0 = (sv0 & kTag) ? 0u : tab[(sv0 & kIdMask) << 8];
     int n = 0;
     if (use_pair) {
         for (int t0 = 0; t0 < pair.n; t0 += 64) {
@@ -374,7 +376,7 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
         if (k0 > 0) {
             const int k = k0 + lane_id();
             const uint32_t sv = k < n_cur ? cl[k] : kTag;
-            e = (sv & kTag) ? 0u : tab[sv & kIdMask];
+            e = (sv & kTag) ? 0u : tab[(sv & kIdMask) << 8];
         }
         const uint32_t cnt = e >> 24;
         const uint32_t d = e & kIdMask;
@@ -698,12 +700,15 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         if (n_act > CAP) sparse = false;
         wave_sync();
     };
-    if (has_wb && L > 0 && word_char(win.at(0))) wb_pass();
+    int c_next = L > 0 ? win.at(0) : 0;
+    if (has_wb && L > 0 && word_char(uint32_t(c_next))) wb_pass();
 
     // two-byte rows of (byte, next byte), first 64 entries loaded a step ahead
     auto load_pair = [&](int c1, int c2) -> RowSlice {
         RowSlice r;
-        const uint32_t* o = g.pair_off + (c1 * 256 + c2);
+        // read-only offsets: scalar loads
+        using CU32 = const __attribute__((address_space(4))) uint32_t;
+        const CU32* o = (CU32*)(g.pair_off) + (c1 * 256 + c2);
         r.b = int(o[0]);
         r.n = int(o[1]) - r.b;
         r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
@@ -718,9 +723,13 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     ctx.c_prev = -1;
     int cached = -1;
     for (int b = 0; b < L; ++b) {
-        const int c = int(win.at(b));
+        // the next byte is read one step ahead (two-byte rows, word boundaries)
+        const int c = c_next;
         RowSlice pair_next{0u, 0, 0};
-        if (have_pair && b + 1 < L) pair_next = load_pair(c, int(win.at(b + 1)));
+        if (b + 1 < L) {
+            c_next = win.at(b + 1);
+            if (have_pair) pair_next = load_pair(c, c_next);
+        }
         // active states including the implicit ones (selector features)
         auto total_active = [&]() {
             int n_tot = n_act;
@@ -829,8 +838,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
-            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(win.at(b + 1))) : word_char(xb);
-            // win.at(b+1) may have moved the window forward: b is not read again
+            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(uint32_t(c_next))) : word_char(xb);
             if (boundary) wb_pass();
         }
         pair = pair_next;
@@ -972,11 +980,14 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             n_act = count_bits(gcur, W);
         }
     };
-    if (has_wb && L > 0 && word_char(win.at(0))) wb_pass();
+    int c_next = L > 0 ? win.at(0) : 0;
+    if (has_wb && L > 0 && word_char(uint32_t(c_next))) wb_pass();
 
     auto load_pair = [&](int c1, int c2) -> RowSlice {
         RowSlice r;
-        const uint32_t* o = g.pair_off + (c1 * 256 + c2);
+        // read-only offsets: scalar loads
+        using CU32 = const __attribute__((address_space(4))) uint32_t;
+        const CU32* o = (CU32*)(g.pair_off) + (c1 * 256 + c2);
         r.b = int(o[0]);
         r.n = int(o[1]) - r.b;
         r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
@@ -991,9 +1002,13 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     ctx.c_prev = -1;
     int cached = -1;
     for (int b = 0; b < L; ++b) {
-        const int c = int(win.at(b));
+        // the next byte is read one step ahead (two-byte rows, word boundaries)
+        const int c = c_next;
         RowSlice pair_next{0u, 0, 0};
-        if (have_pair && b + 1 < L) pair_next = load_pair(c, int(win.at(b + 1)));
+        if (b + 1 < L) {
+            c_next = win.at(b + 1);
+            if (have_pair) pair_next = load_pair(c, c_next);
+        }
         auto total_active = [&]() {
             int n_tot = n_act;
             if (imp >= 0) {
@@ -1065,7 +1080,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
-            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(win.at(b + 1))) : word_char(xb);
+            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(uint32_t(c_next))) : word_char(xb);
             if (boundary) wb_pass();
         }
         pair = pair_next;

@@ -59,6 +59,8 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
         if (int rc = l.ctx->ensure_staging(std::max<int64_t>(cfg.max_batch_bytes, max_req), cfg.max_batch, err))
             return rc;
         if (int rc = l.ctx->ensure_results(cfg.max_batch, err)) return rc;
+        if (cfg.kind == KIND_ENGINE)
+            if (int rc = eng.prepare_dense(*l.ctx, cfg.mode, err)) return rc;
     }
     struct Guard {
         Engine& e;

@@ -142,14 +142,14 @@ int ita_info(ita_handle* h, int64_t* out11);
  * (int32), 4 TKO lim (u64, K*W), 5 TKO leftover offsets (258 int32),
  * 6 TKO leftover (2*n uint32), 7 AS offsets (S*256+1 uint32), 8 AS dst
  * (uint32), 9 init/persistent/accept vectors (3*W u64), 10 device AS
- * table (256*S uint32), 11 its multi-row store (uint32), 12 always-active
- * row offsets (257 int32), 13 always-active rows (uint32), 14 two-byte
- * row offsets (65537 uint32), 15 two-byte rows (uint32), 16 the learned
- * selector's policy table (512 uint32, after ita_set_selector: method of
- * (bucket b, list valid v, byte c) in bits 2*(c%16).. of word
- * (2*b + v)*16 + c/16, bucket = min(15, floor(log2(explicit count + 1)))).
- * Returns the element count; copies min(count, cap) elements when
- * out != NULL. */
+ * table (S*256 uint32, entry of state s on byte c at s*256 + c), 11 its
+ * multi-row store (uint32), 12 always-active row offsets (257 int32),
+ * 13 always-active rows (uint32), 14 two-byte row offsets (65537 uint32),
+ * 15 two-byte rows (uint32), 16 the learned selector's policy table
+ * (512 uint32, after ita_set_selector: method of (bucket b, list valid v,
+ * byte c) in bits 2*(c%16).. of word (2*b + v)*16 + c/16, bucket =
+ * min(15, floor(log2(explicit count + 1)))).  Returns the element count;
+ * copies min(count, cap) elements when out != NULL. */
 int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap);
 
 /* Learned-selector training on the GPU (no counterpart in the reference):

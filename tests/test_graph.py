@@ -183,7 +183,7 @@ def test_merge_rule_ids(tmp_path):
 
 
 def _decode_as_tab(s, S):
-    tab = s.table(10).reshape(256, S)
+    tab = s.table(10).reshape(S, 256).T  # stored state-major
     multi = s.table(11)
     rows = {}
     for c in range(256):
