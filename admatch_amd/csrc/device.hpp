@@ -56,6 +56,7 @@ struct ScanArgs {
     unsigned long long* method_steps;  // [4] steps per method, or null
     uint64_t* dense;          // hashed storage: two W-word bitmaps per wave of the grid, all zero
     const uint32_t* policy;   // learned selector table (2 bits per entry) or null
+    int32_t policy_sparse, policy_dense;  // the table's method for every entry of that list validity, or -1
 };
 
 enum SelectMode : int32_t {

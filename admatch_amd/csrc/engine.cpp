@@ -305,6 +305,8 @@ ScanArgs Engine::make_args(Context& c, const uint8_t* d_text, const int64_t* d_o
     a.mlp = d_mlp_;
     a.mlp_every = mlp_every_ >= 0 ? mlp_every_ : 0;
     a.policy = (d_mlp_ && a.mlp_every == 0) ? d_policy_ : nullptr;
+    a.policy_dense = policy_uniform_[0];
+    a.policy_sparse = policy_uniform_[1];
     a.log_buf = d_log_;
     a.log_cap = log_cap_;
     a.log_top = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrLogTop);
@@ -548,6 +550,7 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
     if (w == nullptr) {
         dfree(d_mlp_);
         dfree(d_policy_);
+        policy_uniform_[0] = policy_uniform_[1] = -1;
         return ADM_OK;
     }
     if (n != kMlpParams) {
@@ -561,6 +564,22 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
     if (!d_policy_) ADM_HIP(hipMalloc(&d_policy_, sizeof(uint32_t) * kPolicyTableWords));
     ADM_HIP(launch_policy(dev_, d_mlp_, d_policy_, nullptr));
     ADM_HIP(hipStreamSynchronize(nullptr));
+    // a list validity whose entries all hold one method needs no lookups
+    std::vector<uint32_t> t(kPolicyTableWords);
+    ADM_HIP(hipMemcpy(t.data(), d_policy_, sizeof(uint32_t) * t.size(), hipMemcpyDeviceToHost));
+    for (int v = 0; v < 2; ++v) {
+        int u = -2;  // entry e = (bucket * 2 + list valid) * 256 + byte
+        for (int e = 0; e < kPolicyTableWords * 16; ++e) {
+            if (((e >> 8) & 1) != v) continue;
+            const int m = int((t[size_t(e >> 4)] >> ((e & 15) * 2)) & 3u);
+            if (u == -2) u = m;
+            else if (u != m) {
+                u = -1;
+                break;
+            }
+        }
+        policy_uniform_[v] = u;
+    }
     return ADM_OK;
 }
 

@@ -169,6 +169,7 @@ class Engine {
     Variant var_;
     float* d_mlp_ = nullptr;
     uint32_t* d_policy_ = nullptr;  // learned selector table (kPolicyTableWords)
+    int32_t policy_uniform_[2] = {-1, -1};  // per list validity: the one method of the table, or -1
     int32_t mlp_every_ = 0;  // 0: policy table
     float* d_log_ = nullptr;
     int64_t log_cap_ = 0;

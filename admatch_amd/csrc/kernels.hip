@@ -360,6 +360,16 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
     // are merged
     const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
     const uint32_t e0 = (sv0 & kTag) ? 0u : tab[(sv0 & kIdMask) << 8];
+    // common case: one chunk of states, one of two-byte row entries, no
+    // multi-destination rows: both candidates per lane in one pass
+    if (n_cur <= 64 && (!use_pair || pair.n <= 64) && !__ballot((e0 >> 24) >= 2)) {
+        const bool okp = use_pair && lane_id() < pair.n;
+        const uint32_t d = e0 & kIdMask;
+        const bool n1 = okp && fut.insert(pair.v);
+        const bool n2 = (e0 >> 24) == 1 && fut.insert(d);
+        const int n = list_append<CAP>(fl, 0, n1, pair.v);
+        return list_append<CAP>(fl, n, n2, d);
+    }
     int n = 0;
     if (use_pair) {
         for (int t0 = 0; t0 < pair.n; t0 += 64) {
@@ -547,11 +557,16 @@ __device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& 
             m = (sparse && ctx.K > ctx.n_act) ? M_AS : M_TKO;
             break;
         case MODE_LEARNED: {
-            // policy table (set whenever this mode is launched): one scalar load
-            using CU32 = const __attribute__((address_space(4))) uint32_t;
-            const CU32* pt = (CU32*)(a.policy);
-            const uint32_t wv = pt[(policy_bucket(n_act) * 2 + int(sparse)) * 16 + (c >> 4)];
-            m = int((wv >> ((c & 15) * 2)) & 3u);
+            // policy table (set whenever this mode is launched): one scalar
+            // load, or none when the table holds one method for every entry
+            // of this list validity
+            m = sparse ? a.policy_sparse : a.policy_dense;
+            if (m < 0) {
+                using CU32 = const __attribute__((address_space(4))) uint32_t;
+                const CU32* pt = (CU32*)(a.policy);
+                const uint32_t wv = pt[uint32_t((policy_bucket(n_act) * 2 + int(sparse)) * 16 + (c >> 4))];
+                m = int((wv >> ((c & 15) * 2)) & 3u);
+            }
             break;
         }
         case MODE_LEARNED_EVERY:
