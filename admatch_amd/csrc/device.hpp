@@ -68,6 +68,7 @@ enum SelectMode : int32_t {
     MODE_LEARNED = 5,  // MLP selector, MODE_HEUR as fallback
     MODE_LOG = 6,      // run all three methods per step and log timings
     MODE_LEARNED_EVERY = 7,  // internal: the network evaluated in the scan every mlp_every steps
+    MODE_LEARNED_AS = 8,     // internal: policy table whose reachable list-valid entries all pick AS
 };
 
 enum Method : int32_t { M_INFA = 0, M_TKO = 1, M_AS = 2 };

@@ -174,9 +174,9 @@ int Engine::load(const char* path, std::string& err) {
 }
 
 size_t Engine::engine_lds_bytes(const Variant& v, int32_t mode) const {
-    // per wave: two lists of cap u32, plus either two W-word bitmaps or two
-    // hash tables of 2*cap u32
-    if (hashed(v, mode)) return size_t(v.wpb) * (size_t(3) * v.cap) * 8;
+    // per wave: two lists of cap u32 and two W-word bitmaps, or two lists of
+    // cap + 64 u32 and two hash tables of 2*cap + 64 u32 (sink entries)
+    if (hashed(v, mode)) return size_t(v.wpb) * (size_t(3) * v.cap + 128) * 8;
     return size_t(v.wpb) * (size_t(2) * g_.W + size_t(v.cap)) * 8;
 }
 
@@ -305,8 +305,8 @@ ScanArgs Engine::make_args(Context& c, const uint8_t* d_text, const int64_t* d_o
     a.mlp = d_mlp_;
     a.mlp_every = mlp_every_ >= 0 ? mlp_every_ : 0;
     a.policy = (d_mlp_ && a.mlp_every == 0) ? d_policy_ : nullptr;
-    a.policy_dense = policy_uniform_[0];
-    a.policy_sparse = policy_uniform_[1];
+    a.policy_dense = policy_uniform(false);
+    a.policy_sparse = policy_uniform(true);
     a.log_buf = d_log_;
     a.log_cap = log_cap_;
     a.log_top = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrLogTop);
@@ -350,12 +350,33 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     return ADM_OK;
 }
 
+int32_t Engine::policy_uniform(bool valid) const {
+    // entries a step can reach under the current list capacity: a valid
+    // list holds at most cap states, an invalid one more
+    if (h_policy_.empty()) return -1;
+    int lim = 0;
+    while (lim < 15 && (2 << lim) - 1 <= var_.cap) ++lim;  // bucket of cap + 1
+    const int b0 = valid ? 0 : lim, b1 = valid ? std::min(lim, 15) : 15;
+    int u = -2;
+    for (int b = b0; b <= b1; ++b)
+        for (int c = 0; c < 256; ++c) {
+            const int e = (b * 2 + int(valid)) * 256 + c;
+            const int m = int((h_policy_[size_t(e >> 4)] >> ((e & 15) * 2)) & 3u);
+            if (u == -2) u = m;
+            else if (u != m) return -1;
+        }
+    return u < 0 ? -1 : u;
+}
+
 int32_t Engine::kernel_mode(int32_t mode) const {
     // the learned kernel reads the policy table; without weights the
-    // heuristic decides, with every > 0 the network runs in the scan
+    // heuristic decides, with every > 0 the network runs in the scan; a
+    // table that picks AS for every list-valid step it can reach runs the
+    // kernel that decides only the other steps
     if (mode != MODE_LEARNED) return mode;
     if (!d_mlp_) return MODE_HEUR;
-    return mlp_every_ > 0 ? MODE_LEARNED_EVERY : MODE_LEARNED;
+    if (mlp_every_ > 0) return MODE_LEARNED_EVERY;
+    return policy_uniform(true) == M_AS ? MODE_LEARNED_AS : MODE_LEARNED;
 }
 
 int Engine::prepare_dense(Context& c, int32_t mode, std::string& err) {
@@ -550,7 +571,7 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
     if (w == nullptr) {
         dfree(d_mlp_);
         dfree(d_policy_);
-        policy_uniform_[0] = policy_uniform_[1] = -1;
+        h_policy_.clear();
         return ADM_OK;
     }
     if (n != kMlpParams) {
@@ -564,22 +585,8 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
     if (!d_policy_) ADM_HIP(hipMalloc(&d_policy_, sizeof(uint32_t) * kPolicyTableWords));
     ADM_HIP(launch_policy(dev_, d_mlp_, d_policy_, nullptr));
     ADM_HIP(hipStreamSynchronize(nullptr));
-    // a list validity whose entries all hold one method needs no lookups
-    std::vector<uint32_t> t(kPolicyTableWords);
-    ADM_HIP(hipMemcpy(t.data(), d_policy_, sizeof(uint32_t) * t.size(), hipMemcpyDeviceToHost));
-    for (int v = 0; v < 2; ++v) {
-        int u = -2;  // entry e = (bucket * 2 + list valid) * 256 + byte
-        for (int e = 0; e < kPolicyTableWords * 16; ++e) {
-            if (((e >> 8) & 1) != v) continue;
-            const int m = int((t[size_t(e >> 4)] >> ((e & 15) * 2)) & 3u);
-            if (u == -2) u = m;
-            else if (u != m) {
-                u = -1;
-                break;
-            }
-        }
-        policy_uniform_[v] = u;
-    }
+    h_policy_.resize(kPolicyTableWords);
+    ADM_HIP(hipMemcpy(h_policy_.data(), d_policy_, sizeof(uint32_t) * h_policy_.size(), hipMemcpyDeviceToHost));
     return ADM_OK;
 }
 

@@ -139,6 +139,9 @@ class Engine {
     int prepare_dense(Context& c, int32_t mode, std::string& err);
     // the engine kernel a public mode runs (the learned mode's depends on the selector)
     int32_t kernel_mode(int32_t mode) const;
+    // the policy table's method for every entry of that list validity that
+    // the current list capacity lets a step reach, or -1
+    int32_t policy_uniform(bool valid) const;
     int policy_table(uint32_t* out, std::string& err);  // kPolicyTableWords u32
     bool has_mlp() const { return d_mlp_ != nullptr; }
     // step logging for the learned selector (MODE_LOG)
@@ -146,7 +149,7 @@ class Engine {
     int64_t fetch_log(float* out, int64_t cap_records, std::string& err);
     size_t engine_lds_bytes(const Variant& v, int32_t mode = MODE_LEARNED) const;
     bool hashed(const Variant& v, int32_t mode) const {
-        return v.store == 1 && (mode == MODE_AS || mode == MODE_LEARNED);
+        return v.store == 1 && (mode == MODE_AS || mode == MODE_LEARNED || mode == MODE_LEARNED_AS);
     }
     size_t plain_lds_bytes() const;
     int device_cu_count() const { return cu_count_; }
@@ -169,7 +172,7 @@ class Engine {
     Variant var_;
     float* d_mlp_ = nullptr;
     uint32_t* d_policy_ = nullptr;  // learned selector table (kPolicyTableWords)
-    int32_t policy_uniform_[2] = {-1, -1};  // per list validity: the one method of the table, or -1
+    std::vector<uint32_t> h_policy_;  // host copy of the table
     int32_t mlp_every_ = 0;  // 0: policy table
     float* d_log_ = nullptr;
     int64_t log_cap_ = 0;

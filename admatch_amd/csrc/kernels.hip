@@ -50,10 +50,16 @@ struct BitSet {
     __device__ __forceinline__ bool insert(uint32_t d) const { return set_new(bits, d); }
 };
 
+// Each table is followed by 64 sink slots, one per lane, that permanently
+// hold kSink: a lane with nothing to insert compare-and-swaps its sink slot,
+// so inserts need no lane masking.
+constexpr uint32_t kSink = 0xfffffffeu;
+
 template <int H>
 struct HashSet {
     static_assert((H & (H - 1)) == 0, "table size must be a power of two");
     static constexpr bool kBounded = true;
+    static constexpr int kSlots = H + 64;  // u32 per table including the sink slots
     uint32_t* tab;
     __device__ __forceinline__ static uint32_t slot(uint32_t d) {
         return (d * 2654435761u) >> (32 - __builtin_ctz(uint32_t(H)));
@@ -69,6 +75,19 @@ struct HashSet {
         }
         return old == kEmpty;
     }
+    // insert for every lane, d only where pred: true if d was new
+    __device__ __forceinline__ bool insert_if(bool pred, uint32_t d) const {
+        const uint32_t key = pred ? d : kSink;
+        uint32_t i = pred ? slot(d) : uint32_t(H + lane_id());
+        uint32_t old = atomicCAS(&tab[i], kEmpty, key);
+        while (old != kEmpty && old != key) {
+            i = (i + 1) & uint32_t(H - 1);
+            old = atomicCAS(&tab[i], kEmpty, key);
+        }
+        return old == kEmpty;
+    }
+    // the sink slots (once per table)
+    __device__ __forceinline__ void init_sink() const { tab[H + lane_id()] = kSink; }
     __device__ __forceinline__ bool contains(uint32_t d) const {
         uint32_t i = slot(d);
         for (;;) {
@@ -121,6 +140,16 @@ __device__ __forceinline__ int list_append(uint32_t* list, int n, bool pred, uin
         const int idx = n + int(mbcnt(m));
         if (unsigned(idx) < unsigned(CAP)) list[idx] = v;
     }
+    return n + __popcll(m);
+}
+
+// list_append for lists followed by 64 sink entries: every lane stores (the
+// ones without a new entry into their sink entry), so no lane masking.
+template <int CAP>
+__device__ __forceinline__ int list_append_all(uint32_t* list, int n, bool pred, uint32_t v) {
+    const uint64_t m = __ballot(pred);
+    const uint32_t idx = uint32_t(n) + mbcnt(m);
+    list[(pred && idx < uint32_t(CAP)) ? idx : uint32_t(CAP + lane_id())] = v;
     return n + __popcll(m);
 }
 
@@ -365,10 +394,17 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
     if (n_cur <= 64 && (!use_pair || pair.n <= 64) && !__ballot((e0 >> 24) >= 2)) {
         const bool okp = use_pair && lane_id() < pair.n;
         const uint32_t d = e0 & kIdMask;
-        const bool n1 = okp && fut.insert(pair.v);
-        const bool n2 = (e0 >> 24) == 1 && fut.insert(d);
-        const int n = list_append<CAP>(fl, 0, n1, pair.v);
-        return list_append<CAP>(fl, n, n2, d);
+        if constexpr (SET::kBounded) {
+            const bool n1 = fut.insert_if(okp, pair.v);
+            const bool n2 = fut.insert_if((e0 >> 24) == 1, d);
+            const int n = list_append_all<CAP>(fl, 0, n1, pair.v);
+            return list_append_all<CAP>(fl, n, n2, d);
+        } else {
+            const bool n1 = okp && fut.insert(pair.v);
+            const bool n2 = (e0 >> 24) == 1 && fut.insert(d);
+            const int n = list_append<CAP>(fl, 0, n1, pair.v);
+            return list_append<CAP>(fl, n, n2, d);
+        }
     }
     int n = 0;
     if (use_pair) {
@@ -569,6 +605,20 @@ __device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& 
             }
             break;
         }
+        case MODE_LEARNED_AS:
+            // the table picks AS whenever the list is valid
+            if (sparse) {
+                m = M_AS;
+            } else {
+                m = a.policy_dense;
+                if (m < 0) {
+                    using CU32 = const __attribute__((address_space(4))) uint32_t;
+                    const CU32* pt = (CU32*)(a.policy);
+                    const uint32_t wv = pt[uint32_t(policy_bucket(n_act) * 2 * 16 + (c >> 4))];
+                    m = int((wv >> ((c & 15) * 2)) & 3u);
+                }
+            }
+            break;
         case MODE_LEARNED_EVERY:
             // the network itself, re-evaluated every mlp_every steps
             load_stats();
@@ -768,7 +818,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         int next_imp = -1;
         if (!log_step) {
             if (m == M_AS) {
-                n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, BitSet{fut}, fl);
+                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl);
                 fsp = n_fut <= CAP;
                 next_imp = c;
             } else {
@@ -782,7 +832,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             if (sparse && can_as) {
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, BitSet{fut}, fl);
+                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl);
                 fsp = n_fut <= CAP;
                 wave_sync();
                 __builtin_amdgcn_s_waitcnt(0);
@@ -1044,7 +1094,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         int n_fut = 0;
         bool fsp = false;
         if (m == M_AS) {
-            n_fut = as_step<CAP>(g, c, pair, imp >= 0 && have_pair && imp == ctx.c_prev, cl, n_act, hf, fl);
+            n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, hf, fl);
             wave_sync();
             if (n_fut <= CAP) {
                 fsp = true;
@@ -1137,8 +1187,9 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
 
 template <int CAP>
 struct LaneH {
-    // u64 words per wave: two lists of CAP u32, two tables of 2*CAP u32
-    static constexpr size_t words() { return size_t(3) * CAP; }
+    // u64 words per wave: two lists of CAP + 64 u32 and two tables of
+    // 2*CAP + 64 u32 (each followed by its 64 sink entries)
+    static constexpr size_t words() { return size_t(3) * CAP + 128; }
 };
 
 // Build knob: ADM_HASHED_WAVES > 0 asks the compiler to fit that many waves
@@ -1162,13 +1213,15 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
     const int wave = threadIdx.x >> 6;
     uint64_t* base = lds + LaneH<CAP>::words() * wave;
     uint32_t* listA = reinterpret_cast<uint32_t*>(base);
-    uint32_t* listB = listA + CAP;
-    uint32_t* hA = listB + CAP;
-    uint32_t* hB = hA + 2 * CAP;
+    uint32_t* listB = listA + CAP + 64;
+    uint32_t* hA = listB + CAP + 64;
+    uint32_t* hB = hA + HashSet<2 * CAP>::kSlots;
     uint64_t* gA = a.dense + (size_t(blockIdx.x) * WPB + wave) * 2 * size_t(g.W);
     uint64_t* gB = gA + g.W;
     HashSet<2 * CAP>{hA}.clear();
     HashSet<2 * CAP>{hB}.clear();
+    HashSet<2 * CAP>{hA}.init_sink();
+    HashSet<2 * CAP>{hB}.init_sink();
     wave_sync();
     int msteps[3] = {0, 0, 0};
     for (;;) {
@@ -1293,6 +1346,7 @@ hipError_t engine_occ_m(size_t lds, int* blocks) {
         switch (mode) {                                                                 \
             case MODE_AS: return FN<WPB, CAP, MODE_AS, true>(__VA_ARGS__);              \
             case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, true>(__VA_ARGS__);    \
+            case MODE_LEARNED_AS: return FN<WPB, CAP, MODE_LEARNED_AS, true>(__VA_ARGS__); \
             default: return hipErrorInvalidValue;                                       \
         }                                                                               \
     }                                                                                   \
@@ -1305,6 +1359,7 @@ hipError_t engine_occ_m(size_t lds, int* blocks) {
         case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, false>(__VA_ARGS__);       \
         case MODE_LOG: return FN<WPB, CAP, MODE_LOG, false>(__VA_ARGS__);               \
         case MODE_LEARNED_EVERY: return FN<WPB, CAP, MODE_LEARNED_EVERY, false>(__VA_ARGS__); \
+        case MODE_LEARNED_AS: return FN<WPB, CAP, MODE_LEARNED_AS, false>(__VA_ARGS__); \
         default: return hipErrorInvalidValue;                                           \
     }
 template <int WPB, int CAP>

@@ -48,8 +48,8 @@ def test_info_without_gpu(gold_dir):
     assert s.variant() == (4, 1024, 128, 0)
     s.set_variant(4, 256, 128, store=1)
     assert s.variant() == (4, 256, 128, 1)
-    # hashed storage: two lists of cap u32 and two tables of 2*cap u32 per string
-    assert s.info()["engine_lds_bytes"] == 4 * 3 * 256 * 8
+    # hashed storage: two lists of cap + 64 u32 and two tables of 2*cap + 64 u32 per string
+    assert s.info()["engine_lds_bytes"] == 4 * (2 * (256 + 64) + 2 * (512 + 64)) * 4
     s.set_variant(2, 512, 64)
     assert s.info()["engine_lds_bytes"] == lds_bitmaps
     with pytest.raises(E.AdmatchError):
