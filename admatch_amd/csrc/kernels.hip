@@ -1203,13 +1203,11 @@ struct LaneH {
 #define ADM_HASHED_ATTR
 #endif
 
+// The per-byte statistics stay in device memory here (they are read off the
+// AS fast path only), keeping the workgroup's LDS to the waves' own.
 template <int WPB, int CAP, int MODE>
 __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevGraph g, ScanArgs a) {
     extern __shared__ uint64_t lds[];
-    __shared__ int4 s_stats[256];
-    for (int i = threadIdx.x; i < 256; i += 64 * WPB) s_stats[i] = g.sym_stats[i];
-    __syncthreads();
-    g.sym_stats = s_stats;
     const int wave = threadIdx.x >> 6;
     uint64_t* base = lds + LaneH<CAP>::words() * wave;
     uint32_t* listA = reinterpret_cast<uint32_t*>(base);
