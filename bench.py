@@ -55,27 +55,75 @@ def time_mode(s, E, d_buf, d_off, n, kind, mode, steps, warmup):
     return step_ms, kern_ms, res
 
 
-def cpu_baseline(path, inputs, budget_s=12.0):
-    """Single-threaded oracle on a bounded prefix of the same batch."""
+def cpu_baseline(path, inputs, budget_s=12.0, threads=1):
+    """The oracle on a bounded prefix of the same batch (strings split
+    across `threads` threads)."""
     import oracle
 
     o = oracle.OracleNFA(path)
     buf, off = inputs.buf, inputs.offsets
     n = off.size - 1
     # calibrate on a few strings, then size the sample to ~budget_s
-    k = min(16, n)
+    k = min(16 * threads, n)
     t0 = time.perf_counter()
-    o.batch_scan(buf, off[: k + 1], nthreads=1)
+    o.batch_scan(buf, off[: k + 1], nthreads=threads)
     dt = max(time.perf_counter() - t0, 1e-6)
     rate = (off[k] - off[0]) / dt
     want = int(min(n, max(k, budget_s * rate / max(1.0, (off[-1] - off[0]) / n))))
     sub = off[: want + 1]
     t0 = time.perf_counter()
-    o.batch_scan(buf, sub, nthreads=1)
+    o.batch_scan(buf, sub, nthreads=threads)
     dt = time.perf_counter() - t0
     nbytes = int(sub[-1] - sub[0])
-    return {"value": round(nbytes / dt / 1e9, 6), "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"first {want} of {n} strings ({nbytes} B), single-threaded oracle, {dt:.1f} s"}
+    what = "single-threaded oracle" if threads == 1 else f"oracle on {threads} threads"
+    return {"value": round(nbytes / dt / 1e9, 6), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"first {want} of {n} strings ({nbytes} B), {what}, {dt:.1f} s"}
+
+
+def readme_workload(E, weights, steps):
+    """The reference README's benchmark (README.md:13-22): CRS rule group 6
+    (script/run.sh:104) against the data/bing.txt request (593 B after
+    tools/batch.cpp:14-30 joins its lines) replicated 1,024 times. Per-request
+    time = kernel time / 1024 (device-resident inputs) and wall time / 1024
+    (host strings in, rule ids out, PCIe included)."""
+    import gzip
+
+    gold = os.path.join(ROOT, "tests", "golden")
+    path = os.path.join(tempfile.mkdtemp(), "crs_g6.nfa")
+    with gzip.open(os.path.join(gold, "crs_g6.nfa.gz"), "rb") as fi, open(path, "wb") as fo:
+        fo.write(fi.read())
+    z = np.load(os.path.join(gold, "http_inputs.npz"))
+    bing = z["buf"][z["offsets"][0]:z["offsets"][1]].tobytes()
+    with open(os.path.join(gold, "crs_expected.json")) as f:
+        expected = json.load(f)["g6"][0]
+    n = 1024
+    buf, off = E.pack_strings([bing] * n)
+    # ita_batched_scan follows the handle's flags: the learned selector
+    s = E.Scratch(E.AD_MATCHER | E.LEARNED_SELECTOR, path).alloc_scratch()
+    if weights is not None:
+        s.set_selector(weights)
+    d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+    out = {"workload": f"CRS group 6 ({s.info()['states']} states) x data/bing.txt request ({len(bing)} B) x {n}",
+           "published_us_per_request": {"iNFA": 7258, "AdMatch": 62, "hardware": "not stated (NVIDIA, README.md:19-20)"},
+           "us_per_request": {}}
+    for name, kind, mode in (("plain_infa", E.KIND_PLAIN, E.MODE_INFA), ("admatch_rule", E.KIND_ENGINE, E.MODE_ADMATCH),
+                             ("learned", E.KIND_ENGINE, E.MODE_LEARNED)):
+        _, kern_ms, res = time_mode(s, E, d_buf, d_off, n, kind, mode, steps, 1)
+        if any(g != expected for g in E.unpack(*res)):
+            raise SystemExit(f"README workload: {name} results differ from the golden fixture")
+        out["us_per_request"][name] = {"kernel": round(float(np.median(kern_ms)) * 1e3 / n, 3)}
+    # host strings in, rule ids out (the reference's per-string figure)
+    walls = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        got = s.batched_scan_packed(buf, off)
+        walls.append(time.perf_counter() - t0)
+    if any(g != expected for g in E.unpack(*got)):
+        raise SystemExit("README workload: BatchedScan results differ from the golden fixture")
+    out["us_per_request"]["learned"]["wall_incl_pcie"] = round(float(np.median(walls)) * 1e6 / n, 3)
+    s.close()
+    log(f"README workload: {out['us_per_request']}")
+    return out
 
 
 def serving(s, E, synth, args):
@@ -115,6 +163,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests per arrival rate for the serving measurement (0: skip)")
     ap.add_argument("--serve-rates", default="50000,200000,800000")
+    ap.add_argument("--no-readme", dest="readme", action="store_false",
+                    help="skip the reference README workload (CRS group 6 x bing.txt x 1024)")
     ap.add_argument("--variant", default=None,
                     help="wpb,cap,store: engine launch variant (default: the autotuned one)")
     ap.add_argument("--tune", default=None, metavar="PATH",
@@ -227,6 +277,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 4), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic},
     }
+    if args.readme and args.config == "large":
+        line["reference_readme_workload"] = readme_workload(E, np.load(SELECTOR_PATH) if learned_ok else None,
+                                                            max(3, args.steps // 2))
     if args.serve_requests > 0 and args.config == "large":
         line["serving"] = serving(s, E, synth, args)
         mid = line["serving"]["rates"][args.serve_rates.split(",")[len(args.serve_rates.split(",")) // 2]]
@@ -236,6 +289,9 @@ def main():
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
         line["cpu_baseline"] = cpu_baseline(path, w.inputs)
+        threads = max(1, min(16, os.cpu_count() or 1))  # the box's CPU share for one GPU
+        log(f"cpu baseline (oracle, {threads} threads)")
+        line["cpu_baseline_all_cores"] = cpu_baseline(path, w.inputs, budget_s=6.0, threads=threads)
     s.close()
     print(json.dumps(line), flush=True)
 
