@@ -1,7 +1,7 @@
 """GPU parity on rule sets compiled by the regex front end: word-boundary
 transitions, anchored starts and ends, case folding and empty matches at
 scale, every scan path and both engine storages against the oracle (and
-Python's re on a sample)."""
+Python's re on short strings)."""
 import numpy as np
 import pytest
 
@@ -52,7 +52,10 @@ def test_compiled_rules_every_path(tmp_path, seed):
     strs = inputs(seed, 600)
     o = oracle.OracleNFA(path)
     exp = [o.scan(s) for s in strs]
-    assert exp[:100] == expected(pats, strs[:100])  # the oracle agrees with re on a sample
+    # the oracle agrees with re on short strings (nested repetitions make
+    # re backtrack exponentially on long ones)
+    short = [x[:24] for x in strs[:60]]
+    assert [o.scan(x) for x in short] == expected(pats, short)
     buf, off = E.pack_strings(strs)
     for name, flags in PATHS:
         s = E.Scratch(flags, path).alloc_scratch()
