@@ -123,8 +123,8 @@ int Context::ensure_results(int32_t n, std::string& err) {
     return ADM_OK;
 }
 
-int Context::grow_pool(std::string& err) {
-    const int64_t want = pool_cap * 4;
+int Context::grow_pool(int64_t need, std::string& err) {
+    const int64_t want = std::max(pool_cap * 2, need + need / 4);
     drop_graphs();
     dfree(d_pool);
     ADM_HIP(hipMalloc(&d_pool, sizeof(uint32_t) * want));
@@ -512,8 +512,10 @@ int Engine::finish_collect(Context& c, int32_t n, std::vector<int32_t>& out_off,
     std::memcpy(c.stats.method_steps, c.h_ctr + kCtrMethods, sizeof(c.stats.method_steps));
     c.stats.pool_used = (long long)top;
     if (c.h_ctr[kCtrOverflow]) {
-        err = "accepted-state pool overflow";
-        return ADM_ERR_CAPACITY;
+        // pool_top counts every string's accepted states, stored or not
+        if (int rc = c.grow_pool(int64_t(top), err)) return rc;
+        err = "accepted-state pool overflow (grown to fit; scan again)";
+        return ADM_ERR_POOL;
     }
     std::vector<uint32_t> pool(size_t(std::min<unsigned long long>(top, (unsigned long long)c.pool_cap)));
     if (!pool.empty()) {
@@ -552,14 +554,13 @@ int Engine::scan_host(const uint8_t* const* text, const int32_t* len, int32_t n,
         return ADM_ERR_STATE;
     }
     if (int rc = stage(ctx_, text, len, n, err)) return rc;
-    for (int attempt = 0; attempt < 4; ++attempt) {
+    // a pool overflow grows the pool to fit, so the second scan succeeds
+    for (int attempt = 0; attempt < 2; ++attempt) {
         if (int rc = scan_device(ctx_, ctx_.d_text, ctx_.d_off, n, kind, mode, false, err)) return rc;
-        int rc = collect(ctx_, n, out_off, out_ids, err);
-        if (rc != ADM_ERR_CAPACITY) return rc;
-        if (int rg = ctx_.grow_pool(err)) return rg;  // pool overflow: grow and rescan
+        const int rc = collect(ctx_, n, out_off, out_ids, err);
+        if (rc != ADM_ERR_POOL) return rc;
     }
-    err = "accepted-state pool overflow";
-    return ADM_ERR_CAPACITY;
+    return ADM_ERR_POOL;
 }
 
 int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) {

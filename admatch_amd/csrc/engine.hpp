@@ -26,6 +26,7 @@ enum Status : int {
     ADM_ERR_CAPACITY = -5,  // output buffer too small
     ADM_ERR_TOO_LARGE = -6, // rule set does not fit the kernel's LDS budget
     ADM_ERR_BUDGET = -7,    // admission budget exhausted
+    ADM_ERR_POOL = -8,      // accepted-state pool overflowed (grown; scan again)
 };
 
 // Kernel used for a scan: the plain single-kernel iNFA path or the engine
@@ -82,7 +83,7 @@ struct Context {
     void drop_graphs();
     int ensure_results(int32_t n, std::string& err);
     int ensure_staging(int64_t bytes, int32_t n, std::string& err);
-    int grow_pool(std::string& err);
+    int grow_pool(int64_t need, std::string& err);
     int ensure_dense(size_t words, std::string& err);
     size_t device_bytes() const;
 };

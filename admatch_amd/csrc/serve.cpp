@@ -100,7 +100,16 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
                 return ADM_ERR_HIP;
             }
             const int32_t bn = int32_t(l.ids.size());
-            if (int rc = eng.finish_collect(*l.ctx, bn, boff, bids, err)) return rc;
+            const int frc = eng.finish_collect(*l.ctx, bn, boff, bids, err);
+            if (frc == ADM_ERR_POOL) {
+                // the lane's pool was grown to fit: scan the batch again
+                if (int rc = eng.scan_device(*l.ctx, l.ctx->d_text, l.ctx->d_off, bn, KernelKind(cfg.kind), cfg.mode,
+                                             false, err))
+                    return rc;
+                if (int rc = eng.begin_collect(*l.ctx, bn, err)) return rc;
+                continue;
+            }
+            if (frc) return frc;
             const double tdone = now_s();
             for (int32_t k = 0; k < bn; ++k) {
                 const int32_t r = l.ids[size_t(k)];

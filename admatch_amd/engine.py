@@ -34,7 +34,7 @@ KIND_PLAIN, KIND_ENGINE = 0, 1
 MODE_NAMES = {MODE_INFA: "infa", MODE_TKO: "tko", MODE_AS: "as", MODE_ADMATCH: "admatch",
               MODE_HEURISTIC: "heuristic", MODE_LEARNED: "learned", MODE_LOG: "log"}
 
-ITA_OK, ITA_ERR_CAPACITY = 0, -5
+ITA_OK, ITA_ERR_CAPACITY, ITA_ERR_POOL = 0, -5, -8
 MLP_PARAMS = 5443
 LOG_RECORD = 22
 
@@ -137,7 +137,7 @@ class Scratch:
             out = np.zeros(cap, dtype=np.int32)
             n = ctypes.c_int32()
             rc = lib().ita_scan(self._h, text, _p(out), cap, ctypes.byref(n))
-            if rc == ITA_ERR_CAPACITY:
+            if rc == ITA_ERR_CAPACITY and n.value > cap:  # output too small: the full count is in n
                 cap = n.value
                 continue
             _err(rc)
@@ -156,7 +156,7 @@ class Scratch:
         while True:
             ids = np.zeros(cap, dtype=np.int32)
             rc = lib().ita_batched_scan(self._h, _p(ptrs), _p(lens), n, _p(out_off), _p(ids), cap)
-            if rc == ITA_ERR_CAPACITY:
+            if rc == ITA_ERR_CAPACITY and int(out_off[-1]) > cap:  # out_ids too small: offsets hold the total
                 cap = int(out_off[-1])
                 continue
             _err(rc)
@@ -169,16 +169,24 @@ class Scratch:
     # -- device-resident path (bench, serving) ---------------------------
     def scan_device(self, d_text: int, d_offsets: int, n: int, kind: int = KIND_ENGINE, mode: int = MODE_ADMATCH,
                     stream: Optional[int] = None, use_graph: bool = False):
-        _err(lib().ita_scan_device(self._h, d_text, d_offsets, n, kind, mode, stream, int(use_graph)))
+        self._last_scan = (d_text, d_offsets, n, kind, mode, stream, int(use_graph))
+        _err(lib().ita_scan_device(self._h, *self._last_scan))
 
     def collect(self, n: int) -> Tuple[np.ndarray, np.ndarray]:
+        """Results of the last scan_device; a pool overflow (the library grows
+        the pool to fit) repeats that scan once."""
         out_off = np.zeros(n + 1, dtype=np.int32)
         cap = max(4 * n, 1024)
+        rescanned = False
         while True:
             ids = np.zeros(cap, dtype=np.int32)
             rc = lib().ita_collect(self._h, n, _p(out_off), _p(ids), cap)
-            if rc == ITA_ERR_CAPACITY:
+            if rc == ITA_ERR_CAPACITY and int(out_off[-1]) > cap:
                 cap = int(out_off[-1])
+                continue
+            if rc == ITA_ERR_POOL and not rescanned and getattr(self, "_last_scan", None):
+                rescanned = True
+                _err(lib().ita_scan_device(self._h, *self._last_scan))
                 continue
             _err(rc)
             return out_off, ids[: out_off[-1]]

@@ -30,6 +30,9 @@ extern "C" {
 #define ITA_ERR_CAPACITY (-5)
 #define ITA_ERR_TOO_LARGE (-6)
 #define ITA_ERR_BUDGET (-7)
+/* ita_collect: the scan's accepted states overflowed the device pool; the
+ * pool has been grown to fit them, so the same scan repeated succeeds. */
+#define ITA_ERR_POOL (-8)
 
 /* Flag bits; values 1..32 are those of the reference, include/ITA_regex.h:10-24. */
 #define ITA_INFA_KERNEL 1

@@ -102,3 +102,28 @@ def test_hashed_synthetic_configs(workloads, cfg, n):
             got = E.unpack(*s.collect(off.size - 1))
             assert got == exp, (cap, mode, first_diff(got, exp))
     s.close()
+
+
+def test_pool_growth(tmp_path):
+    # 2,000 always-accepting states per string overflow the initial
+    # accepted-state pool (8 per string): the library grows it to fit and the
+    # scan is repeated, through every entry point
+    S = 2001
+    lines = [str(S)] + [f"{i} : initial\n{i} -> {i} : 0|255\n{i} : accepting {1 + i % 50}" for i in range(1, S)]
+    lines += ["0 -> 0 : 97"]
+    path = str(tmp_path / "many.nfa")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    strs = [b"", b"a", b"xyz" * 50] * 22
+    exp = [list(range(1, 51))] * len(strs)
+    assert oracle.OracleNFA(path).batch_scan(*E.pack_strings(strs)) == exp
+    for store in (0, 1):
+        s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+        s.set_variant(2, 256, 64, store)
+        assert s.batched_scan(strs) == exp, store
+        buf, off = E.pack_strings(strs)
+        d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+        for mode in (E.MODE_AS, E.MODE_HEURISTIC):
+            s.scan_device(d_buf.ptr, d_off.ptr, len(strs), E.KIND_ENGINE, mode)
+            assert E.unpack(*s.collect(len(strs))) == exp, (store, mode)
+        s.close()
