@@ -30,6 +30,7 @@ struct DevGraph {
     const uint64_t* init_explicit;  // [W] initial states that are not always active
     const int4* sym_stats;    // [256]: |T_c|, k_c, |leftover_c|, start in alw_dst
     int32_t wb_begin, wb_count;  // word-boundary slice of list
+    int32_t wb_parallel;         // no boundary destination is a boundary source
 };
 
 // Per-launch arguments of the batched engine.

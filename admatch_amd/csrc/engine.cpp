@@ -284,6 +284,7 @@ int Engine::upload(std::string& err) {
 #undef ADM_PUT
     dev_.wb_begin = g_.sym_off[kWordBoundary];
     dev_.wb_count = g_.wb_count();
+    dev_.wb_parallel = g_.wb_chain_free() ? 1 : 0;
     dev_ready_ = true;
     return ADM_OK;
 }

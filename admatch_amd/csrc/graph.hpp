@@ -15,6 +15,8 @@
 #include <array>
 #include <cstdint>
 #include <string>
+#include <algorithm>
+#include <iterator>
 #include <vector>
 
 namespace admatch {
@@ -87,6 +89,20 @@ struct Graph {
     int64_t n_persis_lines = 0, n_bad_lines = 0, n_raw_transitions = 0;
 
     int32_t wb_count() const { return sym_off[kWordBoundary + 1] - sym_off[kWordBoundary]; }
+    // true when no word-boundary destination is a word-boundary source: the
+    // in-place pass in (src, dst) order then equals one parallel pass
+    bool wb_chain_free() const {
+        std::vector<uint32_t> src, dst;
+        for (int32_t t = sym_off[kWordBoundary]; t < sym_off[kWordBoundary + 1]; ++t) {
+            src.push_back(list[size_t(t)].src);
+            dst.push_back(list[size_t(t)].dst);
+        }
+        std::sort(src.begin(), src.end());
+        std::sort(dst.begin(), dst.end());
+        std::vector<uint32_t> both;
+        std::set_intersection(src.begin(), src.end(), dst.begin(), dst.end(), std::back_inserter(both));
+        return both.empty();
+    }
     uint64_t content_hash() const;
 };
 

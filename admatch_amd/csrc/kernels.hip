@@ -240,13 +240,20 @@ __global__ __launch_bounds__(BS) void plain_infa_kernel(DevGraph g, ScanArgs a) 
     const uint8_t* x = a.text + beg;
     const int wb_b = soff[256], wb_n = soff[257] - soff[256];
     auto wb_pass = [&](uint64_t* v) {
-        // in-place pass in (src,dst) order on one thread: the
-        // thread_count == 1 semantics of src/iNFA_kernel.cu:145-164
-        if (tid == 0)
+        if (g.wb_parallel) {
+            // no destination is a source: the pass equals one parallel pass
+            for (int t = tid; t < wb_n; t += BS) {
+                const uint2 e = g.list[wb_b + t];
+                if (test_bit(v, e.x)) set_bit_atomic(v, e.y);
+            }
+        } else if (tid == 0) {
+            // in-place pass in (src,dst) order on one thread: the
+            // thread_count == 1 semantics of src/iNFA_kernel.cu:145-164
             for (int t = 0; t < wb_n; ++t) {
                 const uint2 e = g.list[wb_b + t];
                 if (test_bit(v, e.x)) v[e.y >> 6] |= 1ull << (e.y & 63);
             }
+        }
         __syncthreads();
     };
     if (wb_n > 0 && L > 0 && word_char(x[0])) wb_pass(cur);
@@ -744,9 +751,24 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     };
 
     auto wb_pass = [&]() {
+        materialize();
+        if (g.wb_parallel) {
+            // no destination is a source: one parallel pass, the wave's lanes
+            // over the edges
+            for (int t0 = 0; t0 < g.wb_count; t0 += 64) {
+                const int t = t0 + lane_id();
+                const bool ok = t < g.wb_count;
+                const uint2 e = ok ? g.list[g.wb_begin + t] : make_uint2(0u, 0u);
+                const bool nw = ok && test_bit(cur, e.x) && set_new(cur, e.y);
+                n_act = list_append<CAP>(cl, n_act, nw, e.y);
+            }
+            n_act = uniform(n_act);
+            if (n_act > CAP) sparse = false;
+            wave_sync();
+            return;
+        }
         // in place, list order, one lane (thread_count == 1 semantics of
         // src/iNFA_kernel.cu:145-164)
-        materialize();
         int n = n_act;
         if (lane_id() == 0) {
             for (int t = 0; t < g.wb_count; ++t) {
@@ -1020,9 +1042,47 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         imp = -1;
     };
     auto wb_pass = [&]() {
+        materialize();
+        if (g.wb_parallel) {
+            // no destination is a source: one parallel pass.  The edges whose
+            // source is active bound the states it adds; the list form is kept
+            // when they fit.
+            int hits = CAP + 1;
+            if (sparse) {
+                hits = 0;
+                for (int t0 = 0; t0 < g.wb_count; t0 += 64) {
+                    const int t = t0 + lane_id();
+                    const bool ok = t < g.wb_count;
+                    const uint2 e = ok ? g.list[g.wb_begin + t] : make_uint2(0u, 0u);
+                    hits += __popcll(__ballot(ok && hc.contains(e.x)));
+                }
+            }
+            if (n_act + hits <= CAP) {
+                for (int t0 = 0; t0 < g.wb_count; t0 += 64) {
+                    const int t = t0 + lane_id();
+                    const bool ok = t < g.wb_count;
+                    const uint2 e = ok ? g.list[g.wb_begin + t] : make_uint2(0u, 0u);
+                    const bool nw = ok && hc.contains(e.x) && hc.insert(e.y);
+                    n_act = list_append<CAP>(cl, n_act, nw, e.y);
+                }
+                n_act = uniform(n_act);
+                wave_sync();
+            } else {
+                to_dense();
+                for (int t0 = 0; t0 < g.wb_count; t0 += 64) {
+                    const int t = t0 + lane_id();
+                    if (t < g.wb_count) {
+                        const uint2 e = g.list[g.wb_begin + t];
+                        if (test_bit(gcur, e.x)) set_bit_atomic(gcur, e.y);
+                    }
+                }
+                dense_sync();
+                n_act = count_bits(gcur, W);
+            }
+            return;
+        }
         // in place, list order, one lane (thread_count == 1 semantics of
         // src/iNFA_kernel.cu:145-164)
-        materialize();
         if (sparse && n_act + g.wb_count <= CAP) {
             int n = n_act;
             if (lane_id() == 0) {

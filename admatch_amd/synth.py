@@ -292,6 +292,32 @@ def make_ruleset(cfg: GrammarConfig, seed: int) -> RuleSet:
     return RuleSet(rules=rules, nfa_text=text, state_count=S, gap_rules=gaps)
 
 
+# ---------------------------------------------------------------- front-end patterns
+def pattern_text(node, rng=None, boundary_frac: float = 0.0) -> bytes:
+    """The rule as a pattern in the regex front end's syntax
+    (admatch_amd/regex.py): dot-all, and with probability boundary_frac a
+    \\b before a leading / after a trailing word character."""
+    body = regex_source(node)
+    if rng is not None and isinstance(node, Cat) and node.items:
+        word = frozenset(b"abcdefghijklmnopqrstuvwxyzABCDEFGHIJKLMNOPQRSTUVWXYZ0123456789_")
+        first, last = node.items[0], node.items[-1]
+        if isinstance(first, Cat) and first.items and isinstance(first.items[0], Sym) and \
+                first.items[0].syms <= word and rng.random() < boundary_frac:
+            body = b"\\b" + body
+        if isinstance(last, Cat) and last.items and isinstance(last.items[-1], Sym) and \
+                last.items[-1].syms <= word and rng.random() < boundary_frac:
+            body = body + b"\\b"
+    return b"(?s)" + body
+
+
+def frontend_patterns(n_rules: int, seed: int, boundary_frac: float = 0.3) -> List[bytes]:
+    """Patterns of the medium grammar (20% case-insensitive literals) with
+    word boundaries, for the regex front end."""
+    rs = make_ruleset(dataclasses.replace(grammar_for("medium"), n_rules=n_rules), seed)
+    rng = np.random.default_rng(seed + 1)
+    return [pattern_text(r, rng, boundary_frac) for r in rs.rules]
+
+
 # ---------------------------------------------------------------- sampling
 def sample_match(node, rng, max_gap: int = 24) -> bytes:
     """A random member of the rule's language (used to plant matches)."""

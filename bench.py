@@ -126,6 +126,43 @@ def readme_workload(E, weights, steps):
     return out
 
 
+def frontend_workload(E, synth, weights, steps):
+    """Rules written as patterns and compiled by the regex front end: 1,000
+    patterns of the medium grammar with word boundaries, scanned over the
+    medium workload's 16,384 strings (plain iNFA vs the learned engine)."""
+    from admatch_amd import regex
+
+    pats = synth.frontend_patterns(1000, seed=11)
+    t0 = time.perf_counter()
+    text, states = regex.compile_rules(pats, header="front-end workload")
+    compile_s = time.perf_counter() - t0
+    path = os.path.join(tempfile.mkdtemp(), "frontend.nfa")
+    with open(path, "w") as f:
+        f.write(text)
+    w = synth.make_workload("medium")
+    n, nbytes = w.inputs.n, int(w.inputs.offsets[-1])
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    info = s.info()
+    if weights is not None:
+        s.set_selector(weights)
+    d_buf, d_off = E.DeviceArray(w.inputs.buf), E.DeviceArray(w.inputs.offsets)
+    out = {"workload": f"{len(pats)} patterns ({info['states']} states, {info['wb_transitions']} word-boundary "
+                       f"transitions) x {n} strings ({nbytes} B)", "compile_s": round(compile_s, 3), "gbs": {}}
+    ref = None
+    for name, kind, mode in (("plain_infa", E.KIND_PLAIN, E.MODE_INFA), ("as", E.KIND_ENGINE, E.MODE_AS),
+                             ("learned", E.KIND_ENGINE, E.MODE_LEARNED)):
+        _, kern_ms, res = time_mode(s, E, d_buf, d_off, n, kind, mode, steps, 1)
+        res = E.unpack(*res)
+        if ref is None:
+            ref = res
+        elif res != ref:
+            raise SystemExit(f"front-end workload: {name} disagrees with plain iNFA")
+        out["gbs"][name] = round(nbytes / (float(np.median(kern_ms)) * 1e-3) / 1e9, 4)
+    s.close()
+    log(f"front-end workload: {out}")
+    return out
+
+
 def serving(s, E, synth, args):
     """Continuous-batching request loop on the large rule set: Poisson
     arrivals, lognormal lengths (median 800 B, max 16 KB)."""
@@ -163,6 +200,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests per arrival rate for the serving measurement (0: skip)")
     ap.add_argument("--serve-rates", default="50000,200000,800000")
+    ap.add_argument("--no-frontend", dest="frontend", action="store_false",
+                    help="skip the regex front-end workload (1,000 compiled patterns x 16,384 strings)")
     ap.add_argument("--no-readme", dest="readme", action="store_false",
                     help="skip the reference README workload (CRS group 6 x bing.txt x 1024)")
     ap.add_argument("--variant", default=None,
@@ -279,6 +318,9 @@ def main():
     }
     if args.readme and args.config == "large":
         line["reference_readme_workload"] = readme_workload(E, np.load(SELECTOR_PATH) if learned_ok else None,
+                                                            max(3, args.steps // 2))
+    if args.frontend and args.config == "large":
+        line["regex_frontend_workload"] = frontend_workload(E, synth, np.load(SELECTOR_PATH) if learned_ok else None,
                                                             max(3, args.steps // 2))
     if args.serve_requests > 0 and args.config == "large":
         line["serving"] = serving(s, E, synth, args)
