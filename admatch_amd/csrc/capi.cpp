@@ -20,6 +20,11 @@ struct ita_handle {
     std::vector<int32_t> off, ids;  // reused result buffers
 };
 
+struct ita_streams {
+    ita_handle* h = nullptr;
+    admatch::StreamSet set;
+};
+
 namespace {
 thread_local std::string g_err;
 
@@ -177,6 +182,55 @@ int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids
     const int64_t total = h->off.back();
     if (total > cap) return fail(ITA_ERR_CAPACITY, "out_ids too small");
     if (total) std::memcpy(out_ids, h->ids.data(), sizeof(int32_t) * size_t(total));
+    return ITA_OK;
+}
+
+int ita_stream_create(ita_handle* h, int32_t n_streams, ita_streams** out) {
+    if (int rc = check(h)) return rc;
+    if (!out) return fail(ITA_ERR_ARG, "null output");
+    auto* s = new ita_streams;
+    s->h = h;
+    std::string err;
+    if (int rc = h->eng.stream_create(n_streams, s->set, err)) {
+        delete s;
+        return fail(rc, err);
+    }
+    *out = s;
+    return ITA_OK;
+}
+
+int ita_stream_reset(ita_streams* s, int32_t stream) {
+    if (!s) return fail(ITA_ERR_ARG, "null stream set");
+    std::string err;
+    if (int rc = s->h->eng.stream_reset(s->set, stream, err)) return fail(rc, err);
+    return ITA_OK;
+}
+
+int ita_stream_scan(ita_streams* s, const uint8_t* const* chunks, const int32_t* len, const int32_t* stream_ids,
+                    int32_t n, int32_t* out_offsets, int32_t* out_ids, int64_t cap) {
+    if (!s) return fail(ITA_ERR_ARG, "null stream set");
+    ita_handle* h = s->h;
+    if (n < 0 || (n > 0 && (!out_offsets || (cap > 0 && !out_ids)))) return fail(ITA_ERR_ARG, "bad output buffers");
+    std::string err;
+    int rc = h->eng.stream_scan(s->set, admatch::KernelKind(h->kind), h->mode, chunks, len, stream_ids, n, h->off,
+                                h->ids, err);
+    if (rc) return fail(rc, err);
+    if (out_offsets) std::memcpy(out_offsets, h->off.data(), sizeof(int32_t) * (size_t(n) + 1));
+    const int64_t total = h->off.empty() ? 0 : h->off.back();
+    if (total > cap) return fail(ITA_ERR_CAPACITY, "out_ids too small (the streams have advanced: ita_last_results)");
+    if (total) std::memcpy(out_ids, h->ids.data(), sizeof(int32_t) * size_t(total));
+    return ITA_OK;
+}
+
+int64_t ita_last_results(ita_handle* h, int32_t* out_ids, int64_t cap) {
+    if (int rc = check(h)) return rc;
+    const int64_t total = h->off.empty() ? 0 : h->off.back();
+    if (out_ids && cap > 0) std::memcpy(out_ids, h->ids.data(), sizeof(int32_t) * size_t(std::min(total, cap)));
+    return total;
+}
+
+int ita_stream_destroy(ita_streams* s) {
+    delete s;
     return ITA_OK;
 }
 

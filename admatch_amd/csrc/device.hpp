@@ -57,6 +57,15 @@ struct ScanArgs {
     unsigned long long* method_steps;  // [4] steps per method, or null
     uint64_t* dense;          // hashed storage: two W-word bitmaps per wave of the grid, all zero
     const uint32_t* policy;   // learned selector table (2 bits per entry) or null
+    // streaming (LDS-storage engine only): string i continues stream
+    // stream_ids[i], whose saved set (W words) and flags are read from that
+    // slot; the set and flags after string i are written to row i of the
+    // out arrays (committed to the slots once the scan has succeeded)
+    const int32_t* stream_ids;
+    const uint64_t* stream_state;
+    const uint8_t* stream_flags;  // bit 0: a set is saved, bit 1: the last byte so far was a word byte
+    uint64_t* stream_out_state;
+    uint8_t* stream_out_flags;
     int32_t policy_sparse, policy_dense;  // the table's method for every entry of that list validity, or -1
 };
 

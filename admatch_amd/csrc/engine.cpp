@@ -68,6 +68,10 @@ void Context::release() {
     dfree(d_off);
     dfree(d_dense);
     dense_words = 0;
+    dfree(d_stream_ids);
+    dfree(d_stream_out);
+    dfree(d_stream_out_flags);
+    stream_n_cap = 0;
     hfree(h_res_count);
     hfree(h_res_start);
     hfree(h_text);
@@ -77,6 +81,26 @@ void Context::release() {
     pool_cap = 0;
     text_cap = 0;
     off_cap = 0;
+}
+
+int Context::ensure_stream(int32_t n, int32_t W, std::string& err) {
+    if (n <= stream_n_cap) return ADM_OK;
+    if (stream) ADM_HIP(hipStreamSynchronize(stream));
+    drop_graphs();
+    dfree(d_stream_ids);
+    dfree(d_stream_out);
+    dfree(d_stream_out_flags);
+    const int32_t cap = std::max<int32_t>(n, 256);
+    ADM_HIP(hipMalloc(&d_stream_ids, sizeof(int32_t) * cap));
+    ADM_HIP(hipMalloc(&d_stream_out, sizeof(uint64_t) * size_t(cap) * size_t(W)));
+    ADM_HIP(hipMalloc(&d_stream_out_flags, size_t(cap)));
+    stream_n_cap = cap;
+    return ADM_OK;
+}
+
+StreamSet::~StreamSet() {
+    dfree(d_state);
+    dfree(d_flags);
 }
 
 int Context::ensure_dense(size_t words, std::string& err) {
@@ -308,6 +332,13 @@ ScanArgs Engine::make_args(Context& c, const uint8_t* d_text, const int64_t* d_o
     a.policy = (d_mlp_ && a.mlp_every == 0) ? d_policy_ : nullptr;
     a.policy_dense = policy_uniform(false);
     a.policy_sparse = policy_uniform(true);
+    if (c.active_stream) {
+        a.stream_ids = c.d_stream_ids;
+        a.stream_state = c.active_stream->d_state;
+        a.stream_flags = c.active_stream->d_flags;
+        a.stream_out_state = c.d_stream_out;
+        a.stream_out_flags = c.d_stream_out_flags;
+    }
     a.log_buf = d_log_;
     a.log_cap = log_cap_;
     a.log_top = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrLogTop);
@@ -337,8 +368,8 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
         return ADM_ERR_STATE;
     }
     int grid = 0;
-    if (int rc = engine_grid(mode, n, &grid, err)) return rc;
-    const bool hs = hashed(var_, mode);
+    const bool hs = hashed(var_, mode) && !c.active_stream;  // streams use LDS storage
+    if (int rc = engine_grid(mode, n, hs, &grid, err)) return rc;
     if (hs) {
         if (c.dense_words < size_t(grid) * var_.wpb * 2 * size_t(g_.W)) {
             err = "dense bitmaps not prepared for this launch";
@@ -347,7 +378,8 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
         a.dense = c.d_dense;
     }
     ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
-    ADM_HIP(launch_engine(var_.wpb, var_.cap, mode, hs, grid, engine_lds_bytes(var_, mode), st, dev_, a));
+    const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
+    ADM_HIP(launch_engine(var_.wpb, var_.cap, mode, hs, grid, lds, st, dev_, a));
     return ADM_OK;
 }
 
@@ -386,18 +418,18 @@ int Engine::prepare_dense(Context& c, int32_t mode, std::string& err) {
     // sized for the largest grid of this variant, so that batches of any size
     // reuse it
     int grid = 0;
-    if (int rc = engine_grid(mode, std::numeric_limits<int32_t>::max(), &grid, err)) return rc;
+    if (int rc = engine_grid(mode, std::numeric_limits<int32_t>::max(), true, &grid, err)) return rc;
     return c.ensure_dense(size_t(grid) * var_.wpb * 2 * size_t(g_.W), err);
 }
 
-int Engine::engine_grid(int32_t mode, int32_t n, int* grid, std::string& err) {
-    const size_t lds = engine_lds_bytes(var_, mode);
+int Engine::engine_grid(int32_t mode, int32_t n, bool hs, int* grid, std::string& err) {
+    const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
     if (lds + sizeof(int4) * 256 > max_lds_) {
         err = "rule set too large for the engine's LDS budget (lower the waves per workgroup)";
         return ADM_ERR_TOO_LARGE;
     }
     int blocks = 0;
-    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, mode, hashed(var_, mode), lds, &blocks));
+    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, mode, hs, lds, &blocks));
     if (blocks <= 0) {
         err = "engine variant does not fit on a CU";
         return ADM_ERR_TOO_LARGE;
@@ -420,7 +452,7 @@ int Engine::scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off,
     if (int rc = c.init(err)) return rc;
     if (int rc = c.ensure_results(n, err)) return rc;
     if (kind == KIND_ENGINE) mode = kernel_mode(mode);
-    if (kind == KIND_ENGINE && n > 0)
+    if (kind == KIND_ENGINE && n > 0 && !c.active_stream)
         if (int rc = prepare_dense(c, mode, err)) return rc;
     hipStream_t st = c.stream;
     ADM_HIP(hipEventRecord(c.ev0, st));
@@ -589,6 +621,85 @@ int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) 
     ADM_HIP(hipStreamSynchronize(nullptr));
     h_policy_.resize(kPolicyTableWords);
     ADM_HIP(hipMemcpy(h_policy_.data(), d_policy_, sizeof(uint32_t) * h_policy_.size(), hipMemcpyDeviceToHost));
+    return ADM_OK;
+}
+
+int Engine::stream_create(int32_t n, StreamSet& ss, std::string& err) {
+    if (!dev_ready_) {
+        err = "rule set not on the device (allocScratch first)";
+        return ADM_ERR_STATE;
+    }
+    if (n <= 0) {
+        err = "bad stream count";
+        return ADM_ERR_ARG;
+    }
+    ss.cap = n;
+    ss.W = g_.W;
+    ADM_HIP(hipMalloc(&ss.d_state, sizeof(uint64_t) * size_t(n) * size_t(g_.W)));
+    ADM_HIP(hipMalloc(&ss.d_flags, size_t(n)));
+    ADM_HIP(hipMemset(ss.d_flags, 0, size_t(n)));
+    ADM_HIP(hipDeviceSynchronize());
+    return ADM_OK;
+}
+
+int Engine::stream_reset(StreamSet& ss, int32_t id, std::string& err) {
+    if (id < 0 || id >= ss.cap) {
+        err = "bad stream id";
+        return ADM_ERR_ARG;
+    }
+    if (ctx_.stream) ADM_HIP(hipStreamSynchronize(ctx_.stream));
+    ADM_HIP(hipMemset(ss.d_flags + id, 0, 1));
+    ADM_HIP(hipDeviceSynchronize());
+    return ADM_OK;
+}
+
+int Engine::stream_scan(StreamSet& ss, KernelKind kind, int32_t mode, const uint8_t* const* text, const int32_t* len,
+                        const int32_t* ids, int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids,
+                        std::string& err) {
+    if (!dev_ready_) {
+        err = "rule set not on the device (allocScratch first)";
+        return ADM_ERR_STATE;
+    }
+    if (ss.W != g_.W || !ss.d_state) {
+        err = "stream set of another rule set";
+        return ADM_ERR_ARG;
+    }
+    if (n < 0 || (n > 0 && !ids)) {
+        err = "bad arguments";
+        return ADM_ERR_ARG;
+    }
+    std::vector<uint8_t> seen(size_t(ss.cap), 0);
+    for (int32_t i = 0; i < n; ++i) {
+        if (ids[i] < 0 || ids[i] >= ss.cap) {
+            err = "bad stream id";
+            return ADM_ERR_ARG;
+        }
+        if (seen[size_t(ids[i])]++) {
+            err = "a stream appears twice in one call";
+            return ADM_ERR_ARG;
+        }
+    }
+    if (int rc = stage(ctx_, text, len, n, err)) return rc;
+    if (int rc = ctx_.ensure_stream(std::max(n, 1), g_.W, err)) return rc;
+    if (n > 0)
+        ADM_HIP(hipMemcpyAsync(ctx_.d_stream_ids, ids, sizeof(int32_t) * size_t(n), hipMemcpyHostToDevice,
+                               ctx_.stream));
+    // the engine kernel with LDS storage; a plain-kernel handle scans with the
+    // engine's iNFA mode
+    const int32_t m = kind == KIND_PLAIN ? MODE_INFA : mode;
+    ctx_.active_stream = &ss;
+    int rc = ADM_ERR_POOL;
+    for (int attempt = 0; attempt < 2 && rc == ADM_ERR_POOL; ++attempt) {
+        rc = scan_device(ctx_, ctx_.d_text, ctx_.d_off, n, KIND_ENGINE, m, false, err);
+        if (rc) break;
+        rc = collect(ctx_, n, out_off, out_ids, err);
+    }
+    ctx_.active_stream = nullptr;
+    if (rc) return rc;
+    // the scan succeeded: its sets become the streams' saved sets
+    ADM_HIP(launch_stream_commit(ctx_.d_stream_ids, n, g_.W, ctx_.d_stream_out, ctx_.d_stream_out_flags, ss.d_state,
+                                 ss.d_flags, ctx_.stream));
+    ADM_HIP(hipStreamSynchronize(ctx_.stream));
     return ADM_OK;
 }
 

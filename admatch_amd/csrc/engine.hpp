@@ -52,6 +52,15 @@ struct ScanStats {
 // Everything one in-flight scan needs besides the rule set: a stream,
 // result buffers, counters, host/device staging and captured graphs.
 // Contexts of one engine may run concurrently on the device.
+// Device state of a set of streams (each a string scanned in chunks).
+struct StreamSet {
+    int32_t cap = 0;               // streams
+    int32_t W = 0;                 // u64 words per saved set
+    uint64_t* d_state = nullptr;   // [cap][W]
+    uint8_t* d_flags = nullptr;    // [cap], 0 = nothing saved
+    ~StreamSet();
+};
+
 struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
@@ -74,6 +83,14 @@ struct Context {
     int32_t off_cap = 0;
     uint64_t* d_dense = nullptr;  // hashed storage bitmaps (zero between launches)
     size_t dense_words = 0;
+    // streaming scans: the stream set of the scan in progress, the stream of
+    // each string and the per-string output sets
+    const StreamSet* active_stream = nullptr;
+    int32_t* d_stream_ids = nullptr;
+    uint64_t* d_stream_out = nullptr;
+    uint8_t* d_stream_out_flags = nullptr;
+    int32_t stream_n_cap = 0;
+    int ensure_stream(int32_t n, int32_t W, std::string& err);
     std::map<std::tuple<const void*, const void*, int32_t, int32_t, int32_t>, hipGraphExec_t> graphs;
     ScanStats stats;
 
@@ -136,6 +153,13 @@ class Engine {
     void set_variant(const Variant& v);
     const Variant& variant() const { return var_; }
     int set_mlp(const float* w, int32_t n, int32_t every, std::string& err);
+    // streaming: chunk i continues stream ids[i] (distinct within a call);
+    // results are each stream's accepted rules as if it ended after the chunk
+    int stream_create(int32_t n, StreamSet& ss, std::string& err);
+    int stream_reset(StreamSet& ss, int32_t id, std::string& err);
+    int stream_scan(StreamSet& ss, KernelKind kind, int32_t mode, const uint8_t* const* text, const int32_t* len,
+                    const int32_t* ids, int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids,
+                    std::string& err);
     // HBM bitmaps of the hashed storage for context c (no-op for LDS storage)
     int prepare_dense(Context& c, int32_t mode, std::string& err);
     // the engine kernel a public mode runs (the learned mode's depends on the selector)
@@ -157,7 +181,7 @@ class Engine {
     size_t table_bytes() const { return table_bytes_; }
 
    private:
-    int engine_grid(int32_t mode, int32_t n, int* grid, std::string& err);
+    int engine_grid(int32_t mode, int32_t n, bool hashed_storage, int* grid, std::string& err);
     int launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
                std::string& err);
     ScanArgs make_args(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, int32_t mode);

@@ -706,10 +706,16 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     uint64_t* fut = bitsB;
     uint32_t* cl = listA;
     uint32_t* fl = listB;
+    // streaming: the string continues a stream whose state and last byte
+    // class were saved by the previous chunk
+    const int slot = a.stream_ids != nullptr ? a.stream_ids[sid] : -1;
+    const uint32_t sflags = slot >= 0 ? uint32_t(a.stream_flags[slot]) : 0u;
+    const uint64_t* saved = slot >= 0 ? a.stream_state + size_t(slot) * W : nullptr;
     // invariant: both bitmaps are zero here.  The always-active states
     // (initial and persistent) start implicit; the other initial states are
-    // listed.
-    for (int j = lane_id(); j < W; j += 64) cur[j] = g.init_explicit[j];
+    // listed.  A continued stream starts from its saved set, all listed.
+    const uint64_t* start = (sflags & 1u) ? saved : g.init_explicit;
+    for (int j = lane_id(); j < W; j += 64) cur[j] = start[j];
     wave_sync();
     int n_act = uniform(rebuild_list<CAP>(cur, W, cl));
     bool sparse = n_act <= CAP;
@@ -723,7 +729,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     // always-active states alone; imp == -1: none.  Set by AS steps;
     // everything that needs the full set (other methods, the word-boundary
     // pass, acceptance) first calls materialize().
-    int imp = -2;
+    int imp = (sflags & 1u) ? -1 : -2;
     auto alw_range = [&](int cc, int& b0, int& n0) {
         b0 = g.sym_stats[cc].w;
         n0 = (cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total) - b0;
@@ -788,7 +794,9 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         wave_sync();
     };
     int c_next = L > 0 ? win.at(0) : 0;
-    if (has_wb && L > 0 && word_char(uint32_t(c_next))) wb_pass();
+    // a boundary before the first byte: the stream's previous byte (or the
+    // start, a non-word position) against it
+    if (has_wb && L > 0 && word_char(uint32_t(c_next)) != ((sflags & 2u) != 0)) wb_pass();
 
     // two-byte rows of (byte, next byte), first 64 entries loaded a step ahead
     auto load_pair = [&](int c1, int c2) -> RowSlice {
@@ -925,13 +933,25 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
-            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(uint32_t(c_next))) : word_char(xb);
+            // a stream defers its last byte's boundary to the next chunk
+            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(uint32_t(c_next)))
+                                              : (slot < 0 && word_char(xb));
             if (boundary) wb_pass();
         }
         pair = pair_next;
     }
     materialize();
     wave_sync();
+    if (slot >= 0) {
+        // save the set before the end-of-string boundary, then report as if
+        // the stream ended here
+        uint64_t* out = a.stream_out_state + size_t(sid) * W;
+        for (int j = lane_id(); j < W; j += 64) out[j] = cur[j];
+        const bool last_word = L > 0 ? word_char(uint32_t(c_next)) : ((sflags & 2u) != 0);
+        if (lane_id() == 0) a.stream_out_flags[sid] = uint8_t(1u | (last_word ? 2u : 0u));
+        if (has_wb && last_word) wb_pass();
+        wave_sync();
+    }
     emit_accepts(g, a, sid, cur);
     wave_sync();
     clear_bits<CAP>(cur, cl, n_act, sparse, W);
@@ -1339,6 +1359,16 @@ __global__ void fill_i32_kernel(int32_t* p, int32_t v, int64_t n) {
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) p[i] = v;
 }
 
+// streaming: row i of a scan's output sets becomes stream ids[i]'s saved set
+__global__ void stream_commit_kernel(const int32_t* ids, int32_t W, const uint64_t* out_state,
+                                     const uint8_t* out_flags, uint64_t* state, uint8_t* flags) {
+    const int i = blockIdx.x;
+    const int slot = ids[i];
+    for (int j = threadIdx.x; j < W; j += blockDim.x)
+        state[size_t(slot) * W + j] = out_state[size_t(i) * W + j];
+    if (threadIdx.x == 0) flags[slot] = out_flags[i];
+}
+
 // counters layout (int32): [0] queue, [1] overflow, [2..3] pool_top (u64),
 // [4..4+kBins) bin counts, [4+kBins..4+2kBins) bin cursors
 __global__ void bin_count_kernel(const int64_t* offsets, int32_t n, int32_t* bins) {
@@ -1482,6 +1512,13 @@ hipError_t launch_policy(const DevGraph& g, const float* mlp, uint32_t* table, h
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<int32_t*>(table), 0,
                        int64_t(kPolicyWords));
     hipLaunchKernelGGL(policy_kernel, dim3(kPolicyBuckets * 2 * 256), dim3(64), 0, st, g, mlp, table);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_commit(const int32_t* ids, int32_t n, int32_t W, const uint64_t* out_state,
+                                const uint8_t* out_flags, uint64_t* state, uint8_t* flags, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(stream_commit_kernel, dim3(n), dim3(64), 0, st, ids, W, out_state, out_flags, state, flags);
     return hipGetLastError();
 }
 

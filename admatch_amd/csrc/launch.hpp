@@ -14,6 +14,9 @@ hipError_t launch_engine(int wpb, int cap, int mode, bool hashed, int grid, size
 hipError_t engine_occupancy(int wpb, int cap, int mode, bool hashed, size_t lds, int* blocks);
 hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a);
 hipError_t launch_fill(int32_t* p, int32_t v, int64_t n, hipStream_t st);
+// streaming: copies row i of out_state/out_flags to slot ids[i] of state/flags
+hipError_t launch_stream_commit(const int32_t* ids, int32_t n, int32_t W, const uint64_t* out_state,
+                                const uint8_t* out_flags, uint64_t* state, uint8_t* flags, hipStream_t st);
 // learned selector: evaluates the network into the policy table (kPolicyTableWords u32)
 hipError_t launch_policy(const DevGraph& g, const float* mlp, uint32_t* table, hipStream_t st);
 hipError_t launch_order(const int64_t* offsets, int32_t n, int32_t* ctr_bins, int32_t* ctr_cursors, int32_t* order,

@@ -42,7 +42,8 @@ EXPORTED = [
     "ita_create", "ita_alloc", "ita_free", "ita_destroy", "ita_scan", "ita_batched_scan", "ita_scan_device",
     "ita_collect", "ita_last_stats", "ita_set_variant", "ita_get_variant", "ita_set_selector", "ita_set_log",
     "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
-    "ita_selector_forward", "ita_serve_trace",
+    "ita_selector_forward", "ita_serve_trace", "ita_stream_create", "ita_stream_reset", "ita_stream_scan",
+    "ita_stream_destroy", "ita_last_results",
 ]
 
 _LIB = None
@@ -74,6 +75,11 @@ def lib():
             "ita_set_variant": (i32, [vp, i32, i32, i32, i32]),
             "ita_get_variant": (i32, [vp, vp, vp, vp, vp]),
             "ita_set_selector": (i32, [vp, vp, i32, i32]),
+            "ita_stream_create": (i32, [vp, i32, ctypes.POINTER(vp)]),
+            "ita_stream_reset": (i32, [vp, i32]),
+            "ita_stream_scan": (i32, [vp, vp, vp, vp, i32, vp, vp, i64]),
+            "ita_stream_destroy": (i32, [vp]),
+            "ita_last_results": (i64, [vp, vp, i64]),
             "ita_set_log": (i32, [vp, i64, i32]),
             "ita_fetch_log": (i64, [vp, vp, i64]),
             "ita_info": (i32, [vp, vp]),
@@ -256,6 +262,45 @@ class AdScratch(Scratch):
 
     def __init__(self, flags: int, nfa_path: str):
         super().__init__(flags | AD_MATCHER, nfa_path)
+
+
+class Streams:
+    """Strings scanned in chunks over several calls (ita_stream_*): chunk i of
+    a call continues stream ids[i]; each result is that stream's accepted
+    rule ids as if it ended after the chunk."""
+
+    def __init__(self, scratch: "Scratch", n_streams: int):
+        self._scratch = scratch  # keeps the handle alive
+        self._s = ctypes.c_void_p()
+        _err(lib().ita_stream_create(scratch._h, n_streams, ctypes.byref(self._s)))
+
+    def reset(self, stream: int):
+        _err(lib().ita_stream_reset(self._s, stream))
+
+    def scan(self, chunks: Sequence[bytes], ids: Sequence[int]) -> List[List[int]]:
+        buf, offsets = pack_strings(chunks)
+        n = len(chunks)
+        ptrs = (buf.ctypes.data + offsets[:-1]).astype(np.uint64)
+        lens = np.diff(offsets).astype(np.int32)
+        sid = np.ascontiguousarray(ids, dtype=np.int32)
+        out_off = np.zeros(n + 1, dtype=np.int32)
+        cap = max(4 * n, 1024)
+        ids_out = np.zeros(cap, dtype=np.int32)
+        rc = lib().ita_stream_scan(self._s, _p(ptrs), _p(lens), _p(sid), n, _p(out_off), _p(ids_out), cap)
+        if rc == ITA_ERR_CAPACITY:  # the streams advanced: fetch the kept results
+            ids_out = np.zeros(int(out_off[-1]), dtype=np.int32)
+            lib().ita_last_results(self._scratch._h, _p(ids_out), ids_out.size)
+        else:
+            _err(rc)
+        return unpack(out_off, ids_out[: out_off[-1]])
+
+    def close(self):
+        if self._s:
+            lib().ita_stream_destroy(self._s)
+            self._s = ctypes.c_void_p()
+
+    def __del__(self):
+        self.close()
 
 
 def unpack(out_off: np.ndarray, ids: np.ndarray) -> List[List[int]]:

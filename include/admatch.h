@@ -105,6 +105,29 @@ int ita_scan_device(ita_handle* h, const uint8_t* d_text, const int64_t* d_offse
  * to rule ids (layout as ita_batched_scan). */
 int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids, int64_t cap);
 
+/* Streaming scans (no counterpart in the reference, which restarts every
+ * scan from the initial states, src/iNFA_kernel.cu:41-44): a string arrives
+ * as chunks over several calls and the scan carries its state between them.
+ * ita_stream_create allocates device state for n_streams streams of handle h
+ * (valid while h stays allocated); ita_stream_scan scans chunk i as the
+ * continuation of stream stream_ids[i] (each stream at most once per call)
+ * and returns, per chunk, the stream's accepted rule ids as if the stream
+ * ended after that chunk: the chunks of a stream scanned in order give the
+ * results of their concatenation.  The word-boundary decision at a chunk's
+ * last byte is deferred to the next chunk.  On ITA_ERR_CAPACITY the streams
+ * have advanced and out_offsets holds the counts.  ita_stream_reset starts a
+ * stream anew. */
+typedef struct ita_streams ita_streams;
+int ita_stream_create(ita_handle* h, int32_t n_streams, ita_streams** out);
+int ita_stream_reset(ita_streams* s, int32_t stream);
+int ita_stream_scan(ita_streams* s, const uint8_t* const* chunks, const int32_t* len, const int32_t* stream_ids,
+                    int32_t n, int32_t* out_offsets, int32_t* out_ids, int64_t cap);
+int ita_stream_destroy(ita_streams* s);
+/* The rule ids of the handle's last scan or stream scan (all strings,
+ * concatenated; the offsets were returned by that call): copies
+ * min(total, cap) and returns the total. */
+int64_t ita_last_results(ita_handle* h, int32_t* out_ids, int64_t cap);
+
 /* Kernel time (ms, hipEvents) of the last scan and steps per method
  * (iNFA, TKO, AS) of the last engine scan. */
 int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3);
