@@ -163,6 +163,38 @@ def frontend_workload(E, synth, weights, steps):
     return out
 
 
+def streaming_workload(E, path, weights, w, n=16384, parts=4, tune=None):
+    """Streaming scans on the large rule set: the first n strings as n
+    streams, each cut into `parts` chunks scanned in `parts` calls (host
+    strings in, rule ids out) against one BatchedScan of the whole strings."""
+    s = E.Scratch(E.AD_MATCHER | E.LEARNED_SELECTOR, path).alloc_scratch()
+    if tune:
+        s.set_variant(tune["wpb"], tune["cap"], tune.get("plain_bs", 64), tune.get("store", 0))
+    if weights is not None:
+        s.set_selector(weights)
+    strs = [w.inputs.string(i) for i in range(n)]
+    nbytes = sum(len(x) for x in strs)
+    cuts = [[len(x) * k // parts for k in range(parts + 1)] for x in strs]
+    st = E.Streams(s, n)
+    ids = list(range(n))
+    s.batched_scan(strs[:64])  # warm up
+    t0 = time.perf_counter()
+    whole = s.batched_scan(strs)
+    t_whole = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for k in range(parts):
+        got = st.scan([x[c[k]:c[k + 1]] for x, c in zip(strs, cuts)], ids)
+    t_stream = time.perf_counter() - t0
+    if got != whole:
+        raise SystemExit("streaming: chunked results differ from whole-string results")
+    st.close()
+    s.close()
+    out = {"workload": f"{n} strings ({nbytes} B) as {n} streams of {parts} chunks, host strings in",
+           "gbs_streamed": round(nbytes / t_stream / 1e9, 4), "gbs_whole_batched": round(nbytes / t_whole / 1e9, 4)}
+    log(f"streaming: {out}")
+    return out
+
+
 def serving(s, E, synth, args):
     """Continuous-batching request loop on the large rule set: Poisson
     arrivals, lognormal lengths (median 800 B, max 16 KB)."""
@@ -200,6 +232,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=20000,
                     help="requests per arrival rate for the serving measurement (0: skip)")
     ap.add_argument("--serve-rates", default="50000,200000,800000")
+    ap.add_argument("--no-streaming", dest="streaming", action="store_false",
+                    help="skip the streaming measurement (16,384 strings as 4-chunk streams)")
     ap.add_argument("--no-frontend", dest="frontend", action="store_false",
                     help="skip the regex front-end workload (1,000 compiled patterns x 16,384 strings)")
     ap.add_argument("--no-readme", dest="readme", action="store_false",
@@ -322,6 +356,9 @@ def main():
     if args.frontend and args.config == "large":
         line["regex_frontend_workload"] = frontend_workload(E, synth, np.load(SELECTOR_PATH) if learned_ok else None,
                                                             max(3, args.steps // 2))
+    if args.streaming and args.config == "large":
+        line["streaming_workload"] = streaming_workload(E, path, np.load(SELECTOR_PATH) if learned_ok else None, w,
+                                                        tune=tuned)
     if args.serve_requests > 0 and args.config == "large":
         line["serving"] = serving(s, E, synth, args)
         mid = line["serving"]["rates"][args.serve_rates.split(",")[len(args.serve_rates.split(",")) // 2]]
