@@ -574,6 +574,15 @@ __global__ __launch_bounds__(64) void policy_kernel(DevGraph g, const float* mlp
     if (lane_id() == 0) atomicOr(&table[(e >> 4)], uint32_t(m) << ((c & 15) * 2));
 }
 
+// The policy table's method for a step on a bitmap-only set of n_act states.
+__device__ __forceinline__ int dense_policy(const ScanArgs& a, int n_act, int c) {
+    if (a.policy_dense >= 0) return a.policy_dense;
+    using CU32 = const __attribute__((address_space(4))) uint32_t;
+    const CU32* pt = (CU32*)(a.policy);
+    const uint32_t wv = pt[uint32_t(policy_bucket(n_act) * 2 * 16 + (c >> 4))];
+    return int((wv >> ((c & 15) * 2)) & 3u);
+}
+
 // Method for the step on ctx.c.  total_active() counts the active states
 // including the implicitly held ones (selector features).  cached: the last
 // network decision of the every-k-steps mode.
@@ -613,18 +622,9 @@ __device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& 
             break;
         }
         case MODE_LEARNED_AS:
-            // the table picks AS whenever the list is valid
-            if (sparse) {
-                m = M_AS;
-            } else {
-                m = a.policy_dense;
-                if (m < 0) {
-                    using CU32 = const __attribute__((address_space(4))) uint32_t;
-                    const CU32* pt = (CU32*)(a.policy);
-                    const uint32_t wv = pt[uint32_t(policy_bucket(n_act) * 2 * 16 + (c >> 4))];
-                    m = int((wv >> ((c & 15) * 2)) & 3u);
-                }
-            }
+            // the table picks AS whenever the list is valid; a step on a
+            // bitmap-only set reads it in the dense path (dense_policy)
+            m = sparse ? M_AS : M_INFA;
             break;
         case MODE_LEARNED_EVERY:
             // the network itself, re-evaluated every mlp_every steps
@@ -852,6 +852,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 fsp = n_fut <= CAP;
                 next_imp = c;
             } else {
+                if (MODE == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
                 materialize();
                 n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp);
             }
@@ -1197,6 +1198,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             imp = c;
             wave_sync();
         } else {
+            if (MODE == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
             materialize();
             to_dense();
             dense_method(g, m, c, gcur, gfut);
