@@ -1148,65 +1148,6 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     ctx.c_prev = -1;
     int cached = -1;
     for (int b = 0; b < L; ++b) {
-        if constexpr (MODE == MODE_AS || MODE == MODE_LEARNED_AS) {
-            // AS fast path: consecutive steps whose list is valid and holds at
-            // most 64 states, whose two-byte row has at most 64 entries and
-            // whose table rows are single destinations.  Any other step (a
-            // list that overflows included) is left to the general step below
-            // with the loop state untouched; a word boundary after a step is
-            // handled by the general code too.
-            bool boundary = false;
-            while (b < L && sparse && can_as && n_act <= 64) {
-                const bool use_pair = have_pair && imp >= 0;
-                if (use_pair && pair.n > 64) break;
-                const int c = c_next;
-                int cn = c_next;
-                RowSlice pn{0u, 0, 0};
-                if (b + 1 < L) {  // read ahead; committed only with the step
-                    cn = win.at(b + 1);
-                    if (have_pair) pn = load_pair(c, cn);
-                }
-                const uint32_t sv = lane_id() < n_act ? cl[lane_id()] : kTag;
-                const uint32_t e = (sv & kTag) ? 0u : g.as_tab[((sv & kIdMask) << 8) + uint32_t(c)];
-                if (__ballot((e >> 24) >= 2)) break;
-                const bool n1 = hf.insert_if(use_pair && lane_id() < pair.n, pair.v);
-                const bool n2 = hf.insert_if((e >> 24) == 1, e & kIdMask);
-                int n_fut = list_append_all<CAP>(fl, 0, n1, pair.v);
-                n_fut = list_append_all<CAP>(fl, n_fut, n2, e & kIdMask);
-                wave_sync();
-                if (n_fut > CAP) {
-                    hf.clear();
-                    wave_sync();
-                    break;
-                }
-                hc.clear();
-                const HS th = hc;
-                hc = hf;
-                hf = th;
-                uint32_t* tl = cl;
-                cl = fl;
-                fl = tl;
-                n_act = n_fut;
-                imp = c;
-                msteps[2] += 1;
-                c_next = cn;
-                pair = pn;
-                ctx.c_prev = c;
-                ctx.last_m = M_AS;
-                wave_sync();
-                if (has_wb && ((b < L - 1) ? (word_char(uint32_t(c)) != word_char(uint32_t(c_next)))
-                                           : word_char(uint32_t(c)))) {
-                    boundary = true;  // step b is done; its boundary pass follows
-                    break;
-                }
-                ++b;
-            }
-            if (boundary) {
-                wb_pass();
-                continue;
-            }
-            if (b >= L) break;
-        }
         // the next byte is read one step ahead (two-byte rows, word boundaries)
         const int c = c_next;
         RowSlice pair_next{0u, 0, 0};
