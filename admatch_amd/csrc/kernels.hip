@@ -1234,23 +1234,44 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     }
     materialize();
     if (sparse) {
-        // accepted listed states into the (zero) fut bitmap, emitted from there
-        // in ascending order
+        // the accepted listed states, compacted into the free list
         int cnt = 0;
-        for (int k = lane_id(); k < n_act; k += 64) {
-            const uint32_t id = cl[k] & kIdMask;
-            if (test_bit(g.accept, id)) {
-                set_bit_atomic(gfut, id);
-                ++cnt;
-            }
+        for (int k0 = 0; k0 < n_act; k0 += 64) {
+            const int k = k0 + lane_id();
+            const uint32_t id = k < n_act ? (cl[k] & kIdMask) : 0u;
+            cnt = list_append<CAP>(fl, cnt, k < n_act && test_bit(g.accept, id), id);
         }
-        cnt = uniform(wave_sum(cnt));
+        cnt = uniform(cnt);
+        wave_sync();
         if (cnt == 0) {
             if (lane_id() == 0) {
                 a.res_count[sid] = 0;
                 a.res_start[sid] = 0;
             }
+        } else if (cnt <= 64) {
+            // ascending by rank: lane l's id is larger than `rank` others
+            // (the ids are distinct); no bitmap is touched
+            const uint32_t v = lane_id() < cnt ? fl[lane_id()] : 0xffffffffu;
+            int rank = 0;
+            for (int j = 0; j < cnt; ++j) rank += uint32_t(__builtin_amdgcn_readlane(int(v), j)) < v ? 1 : 0;
+            unsigned long long s0 = 0;
+            if (lane_id() == 0) s0 = atomicAdd(a.pool_top, (unsigned long long)cnt);
+            const int64_t start = int64_t(__shfl(static_cast<long long>(s0), 0, 64));
+            int total = cnt;
+            if (start + cnt > a.pool_cap) {
+                if (lane_id() == 0) atomicOr(a.overflow, 1);
+                total = -cnt;  // marks a truncated result
+            } else if (lane_id() < cnt) {
+                a.res_pool[start + rank] = v;
+            }
+            if (lane_id() == 0) {
+                a.res_count[sid] = total;
+                a.res_start[sid] = start;
+            }
         } else {
+            // many accepted states: through the (zero) fut bitmap, emitted
+            // from there in ascending order
+            for (int k = lane_id(); k < cnt; k += 64) set_bit_atomic(gfut, fl[k]);
             dense_sync();
             emit_accepts(g, a, sid, gfut);
             dense_sync();
