@@ -1232,58 +1232,81 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         }
         pair = pair_next;
     }
-    materialize();
+    // accepted states of a valid list: the listed ones and the implicitly
+    // held ones (not in the table), compacted into the free list; up to 64
+    // are emitted by rank without touching the bitmaps
+    int cnt = -1;
     if (sparse) {
-        // the accepted listed states, compacted into the free list
-        int cnt = 0;
+        cnt = 0;
         for (int k0 = 0; k0 < n_act; k0 += 64) {
             const int k = k0 + lane_id();
             const uint32_t id = k < n_act ? (cl[k] & kIdMask) : 0u;
             cnt = list_append<CAP>(fl, cnt, k < n_act && test_bit(g.accept, id), id);
         }
+        if (imp != -1) {
+            int ab = 0, an = g.n_alw_states;
+            const uint32_t* src = g.alw_states;
+            if (imp >= 0) {
+                alw_range(imp, ab, an);
+                src = g.alw_dst;
+            }
+            for (int t0 = 0; t0 < an; t0 += 64) {
+                const int t = t0 + lane_id();
+                const uint32_t id = t < an ? (src[ab + t] & kIdMask) : 0u;
+                cnt = list_append<CAP>(fl, cnt, t < an && test_bit(g.accept, id) && !hc.contains(id), id);
+            }
+        }
         cnt = uniform(cnt);
         wave_sync();
-        if (cnt == 0) {
-            if (lane_id() == 0) {
-                a.res_count[sid] = 0;
-                a.res_start[sid] = 0;
+    }
+    if (cnt == 0) {
+        if (lane_id() == 0) {
+            a.res_count[sid] = 0;
+            a.res_start[sid] = 0;
+        }
+        hc.clear();
+    } else if (cnt > 0 && cnt <= 64) {
+        // ascending by rank: lane l's id is larger than `rank` others (the
+        // ids are distinct)
+        const uint32_t v = lane_id() < cnt ? fl[lane_id()] : 0xffffffffu;
+        int rank = 0;
+        for (int j = 0; j < cnt; ++j) rank += uint32_t(__builtin_amdgcn_readlane(int(v), j)) < v ? 1 : 0;
+        unsigned long long s0 = 0;
+        if (lane_id() == 0) s0 = atomicAdd(a.pool_top, (unsigned long long)cnt);
+        const int64_t start = int64_t(__shfl(static_cast<long long>(s0), 0, 64));
+        int total = cnt;
+        if (start + cnt > a.pool_cap) {
+            if (lane_id() == 0) atomicOr(a.overflow, 1);
+            total = -cnt;  // marks a truncated result
+        } else if (lane_id() < cnt) {
+            a.res_pool[start + rank] = v;
+        }
+        if (lane_id() == 0) {
+            a.res_count[sid] = total;
+            a.res_start[sid] = start;
+        }
+        hc.clear();
+    } else {
+        // many accepted states, or a bitmap-only set: the whole set, then the
+        // accepted states through a bitmap in ascending order
+        materialize();
+        if (sparse) {
+            for (int k = lane_id(); k < n_act; k += 64) {
+                const uint32_t id = cl[k] & kIdMask;
+                if (test_bit(g.accept, id)) set_bit_atomic(gfut, id);
             }
-        } else if (cnt <= 64) {
-            // ascending by rank: lane l's id is larger than `rank` others
-            // (the ids are distinct); no bitmap is touched
-            const uint32_t v = lane_id() < cnt ? fl[lane_id()] : 0xffffffffu;
-            int rank = 0;
-            for (int j = 0; j < cnt; ++j) rank += uint32_t(__builtin_amdgcn_readlane(int(v), j)) < v ? 1 : 0;
-            unsigned long long s0 = 0;
-            if (lane_id() == 0) s0 = atomicAdd(a.pool_top, (unsigned long long)cnt);
-            const int64_t start = int64_t(__shfl(static_cast<long long>(s0), 0, 64));
-            int total = cnt;
-            if (start + cnt > a.pool_cap) {
-                if (lane_id() == 0) atomicOr(a.overflow, 1);
-                total = -cnt;  // marks a truncated result
-            } else if (lane_id() < cnt) {
-                a.res_pool[start + rank] = v;
-            }
-            if (lane_id() == 0) {
-                a.res_count[sid] = total;
-                a.res_start[sid] = start;
-            }
-        } else {
-            // many accepted states: through the (zero) fut bitmap, emitted
-            // from there in ascending order
-            for (int k = lane_id(); k < cnt; k += 64) set_bit_atomic(gfut, fl[k]);
             dense_sync();
             emit_accepts(g, a, sid, gfut);
             dense_sync();
             for (int k = lane_id(); k < n_act; k += 64) gfut[(cl[k] & kIdMask) >> 6] = 0ull;
             dense_sync();
+            hc.clear();
+        } else {
+            emit_accepts(g, a, sid, gcur);
+            dense_sync();
+            for (int j = lane_id(); j < W; j += 64) gcur[j] = 0ull;
+            dense_sync();
         }
-        hc.clear();
-    } else {
-        emit_accepts(g, a, sid, gcur);
-        dense_sync();
-        for (int j = lane_id(); j < W; j += 64) gcur[j] = 0ull;
-        dense_sync();
     }
     wave_sync();
 }
