@@ -702,6 +702,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int L = int(a.offsets[sid + 1] - beg);
+    msteps[3] += L;
     uint64_t* cur = bitsA;
     uint64_t* fut = bitsB;
     uint32_t* cl = listA;
@@ -928,9 +929,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             n_act = uniform(rebuild_list<CAP>(cur, W, cl));
             sparse = n_act <= CAP;
         }
-        msteps[0] += int(m == M_INFA);
-        msteps[1] += int(m == M_TKO);
-        msteps[2] += int(m == M_AS);
+        if (m != M_AS) msteps[m] += 1;  // AS steps: the string length minus these
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
@@ -1003,6 +1002,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int L = int(a.offsets[sid + 1] - beg);
+    msteps[3] += L;
     uint64_t* gcur = gA;
     uint64_t* gfut = gB;
     HS hc{hA}, hf{hB};
@@ -1221,9 +1221,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             imp = -1;
         }
         ctx.c_prev = c;
-        msteps[0] += int(m == M_INFA);
-        msteps[1] += int(m == M_TKO);
-        msteps[2] += int(m == M_AS);
+        if (m != M_AS) msteps[m] += 1;  // AS steps: the string length minus these
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
@@ -1347,7 +1345,7 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
     HashSet<2 * CAP>{hA}.init_sink();
     HashSet<2 * CAP>{hB}.init_sink();
     wave_sync();
-    int msteps[3] = {0, 0, 0};
+    int msteps[4] = {0, 0, 0, 0};  // iNFA, TKO, AS (filled below), bytes scanned
     for (;;) {
         int idx = 0;
         if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
@@ -1356,6 +1354,7 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
         const int sid = a.order ? a.order[idx] : idx;
         scan_string_h<CAP, MODE>(g, a, sid, gA, gB, hA, hB, listA, listB, msteps);
     }
+    msteps[2] = msteps[3] - msteps[0] - msteps[1];
     if (a.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)
             if (msteps[m]) atomicAdd(&a.method_steps[m], (unsigned long long)msteps[m]);
@@ -1378,7 +1377,7 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
     uint32_t* listB = listA + CAP;
     for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
     wave_sync();
-    int msteps[3] = {0, 0, 0};  // steps per method, this wave
+    int msteps[4] = {0, 0, 0, 0};  // iNFA, TKO, AS (filled below), bytes scanned
     for (;;) {
         int idx = 0;
         if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
@@ -1387,6 +1386,7 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
         const int sid = a.order ? a.order[idx] : idx;
         scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, msteps);
     }
+    msteps[2] = msteps[3] - msteps[0] - msteps[1];
     if (a.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)
             if (msteps[m]) atomicAdd(&a.method_steps[m], (unsigned long long)msteps[m]);
