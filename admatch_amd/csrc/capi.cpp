@@ -365,7 +365,8 @@ int ita_selector_forward(const float* features, int64_t n, const float* weights,
 }
 
 int ita_serve_trace(ita_handle* h, int32_t kind, int32_t mode, int32_t lanes, int32_t max_batch,
-                    int64_t max_batch_bytes, int64_t state_budget_bytes, const uint8_t* buf, const int64_t* offsets,
+                    int64_t max_batch_bytes, int64_t state_budget_bytes, int64_t split_bytes, const uint8_t* buf,
+                    const int64_t* offsets,
                     const double* arrival_s, int32_t n, double* latency_s, int32_t* out_offsets, int32_t* out_ids,
                     int64_t cap, double* stats6) {
     if (int rc = check(h)) return rc;
@@ -380,6 +381,7 @@ int ita_serve_trace(ita_handle* h, int32_t kind, int32_t mode, int32_t lanes, in
     cfg.max_batch = max_batch;
     cfg.max_batch_bytes = max_batch_bytes;
     cfg.state_budget_bytes = state_budget_bytes;
+    cfg.split_bytes = split_bytes;
     admatch::ServeResult r;
     std::string err;
     if (int rc = admatch::serve_trace(h->eng, cfg, buf, offsets, arrival_s, n, r, err)) return fail(rc, err);

@@ -2,11 +2,13 @@
 //
 // Requests (strings with arrival times) are replayed in real time.  The loop
 // keeps `lanes` scan contexts, each with its own HIP stream, staging and
-// result buffers; a lane that is free takes every request that has arrived
+// result buffers; a lane that is free takes the requests that have arrived
 // and not started yet (up to max_batch strings / max_batch_bytes, and as
 // long as the device bytes of all in-flight batches stay within the state
-// budget), copies it in, launches the scan and queues the result copies
-// without waiting.  Lanes run concurrently on the device, so requests that
+// budget), copies them in, launches the scan and queues the result copies
+// without waiting.  Short and long requests (split_bytes) wait in separate
+// queues and go to separate batches: a batch lasts as long as its longest
+// string's sequential steps, so short requests are not held by long ones.  Lanes run concurrently on the device, so requests that
 // arrive while others are being scanned start as soon as a lane and budget
 // free up.  Completion is detected by polling each lane's event; a request's
 // latency is its completion time minus its arrival time.
@@ -73,7 +75,11 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
 
     out.latency_s.assign(size_t(n), 0.0);
     std::vector<std::vector<int32_t>> res(static_cast<size_t>(n));
-    std::deque<int32_t> pending;
+    // pending requests by length class (short, long); one class per batch
+    std::deque<int32_t> pending_cls[2];
+    auto cls_of = [&](int32_t r) {
+        return (cfg.split_bytes > 0 && offsets[r + 1] - offsets[r] > cfg.split_bytes) ? 1 : 0;
+    };
     int32_t next = 0, completed = 0;
     int64_t inflight = 0;
     out.batches = 0;
@@ -89,7 +95,10 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
     while (completed < n) {
         bool progressed = false;
         const double t = now_s();
-        while (next < n && arrival_s[next] <= t) pending.push_back(next++);
+        while (next < n && arrival_s[next] <= t) {
+            pending_cls[cls_of(next)].push_back(next);
+            ++next;
+        }
         // completions
         for (auto& l : lanes) {
             if (!l.busy) continue;
@@ -124,7 +133,12 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
         }
         // admission
         for (auto& l : lanes) {
-            if (l.busy || pending.empty()) continue;
+            if (l.busy || (pending_cls[0].empty() && pending_cls[1].empty())) continue;
+            // the class whose oldest request arrived first
+            const int k = pending_cls[0].empty()   ? 1
+                          : pending_cls[1].empty() ? 0
+                          : (pending_cls[0].front() < pending_cls[1].front() ? 0 : 1);
+            std::deque<int32_t>& pending = pending_cls[k];
             int64_t text = 0;
             l.ids.clear();
             while (!pending.empty() && int32_t(l.ids.size()) < cfg.max_batch) {
@@ -162,7 +176,7 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
         for (auto& l : lanes) busy += l.busy;
         out.max_lanes_busy = std::max(out.max_lanes_busy, busy);
         if (!progressed) {
-            if (busy == 0 && pending.empty() && next < n) {
+            if (busy == 0 && pending_cls[0].empty() && pending_cls[1].empty() && next < n) {
                 // idle until the next arrival
                 const double wait = arrival_s[next] - now_s();
                 if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));

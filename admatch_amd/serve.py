@@ -27,7 +27,7 @@ def _sig():
     L = E.lib()
     vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
     L.ita_serve_trace.restype = i32
-    L.ita_serve_trace.argtypes = [vp, i32, i32, i32, i32, i64, i64, vp, vp, vp, i32, vp, vp, vp, i64, vp]
+    L.ita_serve_trace.argtypes = [vp, i32, i32, i32, i32, i64, i64, i64, vp, vp, vp, i32, vp, vp, vp, i64, vp]
     return L
 
 
@@ -38,7 +38,7 @@ def poisson_arrivals(n: int, rate: float, seed: int) -> np.ndarray:
 
 def serve_trace(s: E.Scratch, buf: np.ndarray, offsets: np.ndarray, arrival_s: np.ndarray, kind: int = E.KIND_ENGINE,
                 mode: int = E.MODE_LEARNED, lanes: int = 4, max_batch: int = 4096, max_batch_bytes: int = 8 << 20,
-                state_budget: int = 256 << 20):
+                state_budget: int = 256 << 20, split_bytes: int = 2048):
     buf = np.ascontiguousarray(buf, dtype=np.uint8)
     offsets = np.ascontiguousarray(offsets, dtype=np.int64)
     arrival_s = np.ascontiguousarray(arrival_s, dtype=np.float64)
@@ -49,10 +49,10 @@ def serve_trace(s: E.Scratch, buf: np.ndarray, offsets: np.ndarray, arrival_s: n
     cap = max(4 * n, 1024)
     while True:
         ids = np.zeros(cap, dtype=np.int32)
-        rc = _sig().ita_serve_trace(s._h, kind, mode, lanes, max_batch, max_batch_bytes, state_budget,
+        rc = _sig().ita_serve_trace(s._h, kind, mode, lanes, max_batch, max_batch_bytes, state_budget, split_bytes,
                                     buf.ctypes.data, offsets.ctypes.data, arrival_s.ctypes.data, n,
                                     lat.ctypes.data, out_off.ctypes.data, ids.ctypes.data, cap, stats.ctypes.data)
-        if rc == E.ITA_ERR_CAPACITY:
+        if rc == E.ITA_ERR_CAPACITY and int(out_off[-1]) > cap:
             cap = int(out_off[-1])
             continue
         E._err(rc)

@@ -198,14 +198,15 @@ int ita_selector_forward(const float* features, int64_t n, const float* weights,
  * arrived request not yet started, up to max_batch strings and
  * max_batch_bytes input bytes, while the device bytes of all in-flight
  * batches stay within state_budget_bytes (ITA_ERR_BUDGET if one request
- * alone exceeds it).  Per request: latency_s[i] (completion - arrival) and
+ * alone exceeds it); requests up to split_bytes long and longer ones wait in
+ * separate queues and are batched apart (0: one queue).  Per request: latency_s[i] (completion - arrival) and
  * rule ids in out_ids[out_offsets[i] .. out_offsets[i+1]).  stats[0..5]:
  * wall seconds, batches, mean batch size, peak in-flight device bytes, peak
  * busy lanes, total input bytes. */
 int ita_serve_trace(ita_handle* h, int32_t kind, int32_t mode, int32_t lanes, int32_t max_batch,
-                    int64_t max_batch_bytes, int64_t state_budget_bytes, const uint8_t* buf, const int64_t* offsets,
-                    const double* arrival_s, int32_t n, double* latency_s, int32_t* out_offsets, int32_t* out_ids,
-                    int64_t cap, double* stats6);
+                    int64_t max_batch_bytes, int64_t state_budget_bytes, int64_t split_bytes, const uint8_t* buf,
+                    const int64_t* offsets, const double* arrival_s, int32_t n, double* latency_s,
+                    int32_t* out_offsets, int32_t* out_ids, int64_t cap, double* stats6);
 
 /* Message of this thread's last failed call. */
 const char* ita_last_error(void);
