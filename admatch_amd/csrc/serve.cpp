@@ -8,9 +8,11 @@
 // budget), copies them in, launches the scan and queues the result copies
 // without waiting.  Short and long requests (split_bytes) wait in separate
 // queues and go to separate batches: a batch lasts as long as its longest
-// string's sequential steps, so short requests are not held by long ones.  Lanes run concurrently on the device, so requests that
-// arrive while others are being scanned start as soon as a lane and budget
-// free up.  Completion is detected by polling each lane's event; a request's
+// string's sequential steps, so short requests are not held by long ones.
+// Long batches hold at most half of the lanes (at least one), so a short
+// request always finds a lane within one short batch's time.  Lanes run
+// concurrently on the device, so requests that arrive while others are being
+// scanned start as soon as a lane and budget free up.  Completion is detected by polling each lane's event; a request's
 // latency is its completion time minus its arrival time.
 #include "serve.hpp"
 
@@ -28,6 +30,7 @@ using clk = std::chrono::steady_clock;
 struct Lane {
     std::unique_ptr<Context> ctx;
     bool busy = false;
+    int cls = 0;               // length class of the batch in flight
     std::vector<int32_t> ids;  // request ids of the batch
     int64_t device_bytes = 0;
 };
@@ -80,6 +83,8 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
     auto cls_of = [&](int32_t r) {
         return (cfg.split_bytes > 0 && offsets[r + 1] - offsets[r] > cfg.split_bytes) ? 1 : 0;
     };
+    const int long_lanes = cfg.split_bytes > 0 ? std::max(1, cfg.lanes / 2) : cfg.lanes;
+    int long_busy = 0;
     int32_t next = 0, completed = 0;
     int64_t inflight = 0;
     out.batches = 0;
@@ -127,6 +132,7 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
             }
             completed += bn;
             inflight -= l.device_bytes;
+            long_busy -= l.cls;
             l.busy = false;
             l.ids.clear();
             progressed = true;
@@ -134,9 +140,12 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
         // admission
         for (auto& l : lanes) {
             if (l.busy || (pending_cls[0].empty() && pending_cls[1].empty())) continue;
-            // the class whose oldest request arrived first
-            const int k = pending_cls[0].empty()   ? 1
-                          : pending_cls[1].empty() ? 0
+            // the class whose oldest request arrived first, long ones only
+            // while fewer than long_lanes lanes hold long batches
+            const bool long_ok = !pending_cls[1].empty() && long_busy < long_lanes;
+            if (pending_cls[0].empty() && !long_ok) break;
+            const int k = !long_ok                 ? 0
+                          : pending_cls[0].empty() ? 1
                           : (pending_cls[0].front() < pending_cls[1].front() ? 0 : 1);
             std::deque<int32_t>& pending = pending_cls[k];
             int64_t text = 0;
@@ -154,10 +163,10 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
             const int32_t bn = int32_t(l.ids.size());
             ptrs.resize(size_t(bn));
             lens.resize(size_t(bn));
-            for (int32_t k = 0; k < bn; ++k) {
-                const int32_t r = l.ids[size_t(k)];
-                ptrs[size_t(k)] = buf + offsets[r];
-                lens[size_t(k)] = int32_t(offsets[r + 1] - offsets[r]);
+            for (int32_t j = 0; j < bn; ++j) {
+                const int32_t r = l.ids[size_t(j)];
+                ptrs[size_t(j)] = buf + offsets[r];
+                lens[size_t(j)] = int32_t(offsets[r + 1] - offsets[r]);
             }
             if (int rc = eng.stage(*l.ctx, ptrs.data(), lens.data(), bn, err)) return rc;
             if (int rc = eng.scan_device(*l.ctx, l.ctx->d_text, l.ctx->d_off, bn, KernelKind(cfg.kind), cfg.mode,
@@ -166,6 +175,8 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
             if (int rc = eng.begin_collect(*l.ctx, bn, err)) return rc;
             l.device_bytes = batch_bytes(text, bn);
             inflight += l.device_bytes;
+            l.cls = k;
+            long_busy += k;
             l.busy = true;
             out.batches += 1;
             batch_sum += bn;

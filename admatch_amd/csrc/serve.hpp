@@ -16,8 +16,9 @@ struct ServeConfig {
     int32_t max_batch = 4096;                   // strings per batch
     int64_t max_batch_bytes = int64_t(8) << 20; // input bytes per batch
     int64_t state_budget_bytes = int64_t(256) << 20;  // device bytes of all in-flight batches
-    // requests up to split_bytes and longer ones are batched apart, so short
-    // requests do not wait for long strings (0: one queue)
+    // requests up to split_bytes and longer ones are batched apart and long
+    // batches hold at most half of the lanes, so short requests do not wait
+    // for long strings (0: one queue)
     int64_t split_bytes = 2048;
 };
 

@@ -199,8 +199,9 @@ int ita_selector_forward(const float* features, int64_t n, const float* weights,
  * max_batch_bytes input bytes, while the device bytes of all in-flight
  * batches stay within state_budget_bytes (ITA_ERR_BUDGET if one request
  * alone exceeds it); requests up to split_bytes long and longer ones wait in
- * separate queues and are batched apart (0: one queue).  Per request: latency_s[i] (completion - arrival) and
- * rule ids in out_ids[out_offsets[i] .. out_offsets[i+1]).  stats[0..5]:
+ * separate queues and are batched apart, long batches holding at most half
+ * of the lanes (0: one queue).  Per request: latency_s[i] (completion -
+ * arrival) and rule ids in out_ids[out_offsets[i] .. out_offsets[i+1]).  stats[0..5]:
  * wall seconds, batches, mean batch size, peak in-flight device bytes, peak
  * busy lanes, total input bytes. */
 int ita_serve_trace(ita_handle* h, int32_t kind, int32_t mode, int32_t lanes, int32_t max_batch,

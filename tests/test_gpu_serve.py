@@ -11,15 +11,16 @@ from admatch_amd import serve  # noqa: E402
 import oracle  # noqa: E402
 
 
-@pytest.mark.parametrize("rate,lanes,budget", [(2e5, 4, 256 << 20), (2e6, 3, 64 << 10), (5e4, 1, 256 << 20)])
-def test_mix_equals_alone(workloads, rate, lanes, budget):
+@pytest.mark.parametrize("rate,lanes,budget,split", [(2e5, 4, 256 << 20, 2048), (2e6, 3, 64 << 10, 0),
+                                                     (5e4, 1, 256 << 20, 1024)])
+def test_mix_equals_alone(workloads, rate, lanes, budget, split):
     w, path = workloads("medium", 1500)
     buf, off = w.inputs.buf, w.inputs.offsets
     n = off.size - 1
     s = E.Scratch(E.AD_MATCHER | E.HEURISTIC_SELECTOR, path).alloc_scratch()
     arr = serve.poisson_arrivals(n, rate, seed=1)
     lat, res, info = serve.serve_trace(s, buf, off, arr, mode=E.MODE_HEURISTIC, lanes=lanes, max_batch=256,
-                                       state_budget=budget)
+                                       state_budget=budget, split_bytes=split)
     assert info["batches"] >= 2 and info["peak_inflight_bytes"] <= budget
     assert (lat > 0).all()
     # alone: each request scanned by itself
