@@ -69,6 +69,15 @@ def summarize(lat: np.ndarray, info: dict, n: int) -> dict:
             "mean_batch": round(info["mean_batch"], 1), "peak_lanes_busy": info["peak_lanes_busy"]}
 
 
+def hw_queues_for(lanes: int) -> None:
+    """Give each lane a hardware queue of its own: HIP gives a process
+    GPU_MAX_HW_QUEUES queues (4 by default) and streams beyond that share one,
+    so a short batch would wait behind a long one queued on the same queue.
+    Takes effect only before the HIP runtime starts."""
+    want = lanes + 4  # the engine's own stream, the null stream, spare
+    os.environ["GPU_MAX_HW_QUEUES"] = str(max(int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), want))
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--rates", default="50000,200000,800000")
@@ -76,6 +85,7 @@ def main(argv=None):
     ap.add_argument("--lanes", type=int, default=4)
     ap.add_argument("--plain-requests", type=int, default=4000)
     args = ap.parse_args(argv)
+    hw_queues_for(args.lanes)
     w = synth.make_workload("serve", n=args.requests)
     d = tempfile.mkdtemp()
     path = os.path.join(d, "serve.nfa")

@@ -29,6 +29,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The serving measurement runs four lanes on streams of their own.  HIP gives
+# a process GPU_MAX_HW_QUEUES hardware queues (4 by default) and streams beyond
+# that share them: with the engine's own stream and torch's already holding
+# queues, lanes would pair up and a short batch would wait behind a long one
+# queued on the same hardware queue.  Set before the HIP runtime starts.
+os.environ["GPU_MAX_HW_QUEUES"] = str(max(int(os.environ.get("GPU_MAX_HW_QUEUES", "4")), 8))
+
 HBM_PEAK_GBS = 8000.0
 SELECTOR_PATH = os.path.join(ROOT, "admatch_amd", "data", "selector_weights.npy")
 
