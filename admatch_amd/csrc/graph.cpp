@@ -2,6 +2,7 @@
 #include "graph.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <sstream>
@@ -324,6 +325,15 @@ void build_tko(Graph& g, int32_t threshold) {
     g.tko_loff[kSymbols] = int32_t(g.tko_left.size());
 }
 
+// Entries the two-byte rows may hold: kMaxPairEntries, or less when the
+// environment variable ADM_MAX_PAIR_ENTRIES says so (tests set 0 to run the
+// engine without the rows).
+static size_t pair_limit() {
+    const char* e = std::getenv("ADM_MAX_PAIR_ENTRIES");
+    if (!e || !*e) return kMaxPairEntries;
+    return std::min<size_t>(kMaxPairEntries, size_t(std::strtoull(e, nullptr, 10)));
+}
+
 void build_as(Graph& g) {
     const size_t rows = size_t(g.S) * 256;
     g.as_off.assign(rows + 1, 0);
@@ -411,6 +421,7 @@ void build_as(Graph& g) {
     g.pair_off.assign(size_t(256) * 256 + 1, 0);
     g.pair_dst.clear();
     bool pair_ok = true;
+    const size_t max_pair = pair_limit();
     std::vector<uint32_t> d1;
     for (int c1 = 0; c1 < 256 && pair_ok; ++c1) {
         d1.clear();
@@ -427,7 +438,7 @@ void build_as(Graph& g) {
             }
             std::sort(row.begin(), row.end());
             row.erase(std::unique(row.begin(), row.end()), row.end());
-            if (g.pair_dst.size() + row.size() > kMaxPairEntries) {
+            if (g.pair_dst.size() + row.size() > max_pair) {
                 pair_ok = false;
                 break;
             }

@@ -82,7 +82,9 @@ struct Graph {
     // (the first positions, D1(c1)) are active at the next step whatever
     // else happened, so their rows on the next byte c2 are merged ahead of
     // time: pair_dst[pair_off[c1*256+c2] .. pair_off[c1*256+c2+1]).  Empty
-    // when the merged rows would exceed kMaxPairEntries.
+    // when the merged rows would exceed kMaxPairEntries (or the limit in
+    // ADM_MAX_PAIR_ENTRIES); the engine then lists the first positions
+    // before each AS step.
     std::vector<uint32_t> pair_off, pair_dst;
 
     // parse statistics

@@ -843,6 +843,9 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
         ctx.pos = b;
         bool log_step = false;
         int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
+        // without two-byte rows the first positions of the implicit set are
+        // not reachable from the table: list them before an AS step
+        if (m == M_AS && !have_pair && imp >= 0) materialize();
         if (m == M_AS && !(sparse && can_as)) m = M_INFA;
         bool fsp;
         int n_fut;
@@ -1171,6 +1174,9 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         ctx.pos = b;
         bool log_step = false;
         int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
+        // without two-byte rows the first positions of the implicit set are
+        // not reachable from the table: list them before an AS step
+        if (m == M_AS && !have_pair && imp >= 0) materialize();
         if (m == M_AS && !(sparse && can_as)) m = M_INFA;
         int n_fut = 0;
         bool fsp = false;

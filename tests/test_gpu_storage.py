@@ -127,3 +127,28 @@ def test_pool_growth(tmp_path):
             s.scan_device(d_buf.ptr, d_off.ptr, len(strs), E.KIND_ENGINE, mode)
             assert E.unpack(*s.collect(len(strs))) == exp, (store, mode)
         s.close()
+
+
+@pytest.mark.parametrize("kind,seed", [("random", 3), ("burst", 3)])
+def test_without_pair_rows(tmp_path, monkeypatch, workloads, kind, seed):
+    # rule sets whose two-byte rows exceed the entry limit run without them:
+    # the first positions are listed before each AS step
+    monkeypatch.setenv("ADM_MAX_PAIR_ENTRIES", "0")
+    cases = [make_case(tmp_path, kind, seed)]
+    w, mpath = workloads("medium", 1024)
+    strs = [w.inputs.string(i) for i in range(w.inputs.n)]
+    cases.append((mpath, strs, oracle.OracleNFA(mpath).batch_scan(w.inputs.buf, w.inputs.offsets, nthreads=8)))
+    for path, strs, exp in cases:
+        s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+        assert int(E.lib().ita_table(s._h, 15, None, 0)) == 0
+        buf, off = E.pack_strings(strs)
+        d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+        for wpb, cap, store in ((2, 512, 0), (2, 256, 1)):
+            s.set_variant(wpb, cap, 64, store)
+            for mode in ((E.MODE_AS, E.MODE_LEARNED, E.MODE_HEURISTIC) if store == 0 else HASHED_MODES):
+                print(f"[case] no pair rows {os.path.basename(path)} store={store} mode={E.MODE_NAMES[mode]}",
+                      flush=True)
+                s.scan_device(d_buf.ptr, d_off.ptr, len(strs), E.KIND_ENGINE, mode)
+                got = E.unpack(*s.collect(len(strs)))
+                assert got == exp, (store, mode, first_diff(got, exp))
+        s.close()

@@ -238,3 +238,20 @@ def test_device_as_table(gold_dir, name):
         want = sorted(set().union(*[exp.get((c2, a), set()) for a in first]) - alw_set) if first else []
         got = pdst[poff[c1 * 256 + c2]:poff[c1 * 256 + c2 + 1]]
         assert got.tolist() == want, (c1, c2)
+
+
+def test_pair_row_limit(gold_dir, monkeypatch):
+    # past the entry limit the two-byte rows are dropped whole (the engine
+    # then lists the first positions before each AS step)
+    path = golden_nfa("synth_small.nfa.gz", gold_dir)
+    full = E.Scratch(E.AD_MATCHER, path)
+    assert full.table(15).size > 1 and full.table(14).size == 256 * 256 + 1
+    full.close()
+    monkeypatch.setenv("ADM_MAX_PAIR_ENTRIES", "0")
+    s = E.Scratch(E.AD_MATCHER, path)
+    assert lib_count(s, 14) == 0 and lib_count(s, 15) == 0
+    s.close()
+
+
+def lib_count(s, which):
+    return int(E.lib().ita_table(s._h, which, None, 0))
