@@ -932,7 +932,10 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             n_act = uniform(rebuild_list<CAP>(cur, W, cl));
             sparse = n_act <= CAP;
         }
-        if (m != M_AS) msteps[m] += 1;  // AS steps: the string length minus these
+        // constant indices keep the counters in registers (AS steps: the
+        // string length minus these)
+        msteps[0] += m == M_INFA ? 1 : 0;
+        msteps[1] += m == M_TKO ? 1 : 0;
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
@@ -1205,6 +1208,10 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             wave_sync();
         } else {
             if (MODE == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
+            // bitmap steps counted here, off the AS path (AS steps: the string
+            // length minus these)
+            msteps[0] += m == M_TKO ? 0 : 1;
+            msteps[1] += m == M_TKO ? 1 : 0;
             materialize();
             to_dense();
             dense_method(g, m, c, gcur, gfut);
@@ -1227,7 +1234,6 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             imp = -1;
         }
         ctx.c_prev = c;
-        if (m != M_AS) msteps[m] += 1;  // AS steps: the string length minus these
         ctx.last_m = m;
         if (has_wb) {
             const uint32_t xb = uint32_t(c);
