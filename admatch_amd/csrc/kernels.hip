@@ -396,21 +396,39 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
     // are merged
     const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
     const uint32_t e0 = (sv0 & kTag) ? 0u : tab[(sv0 & kIdMask) << 8];
-    // common case: one chunk of states, one of two-byte row entries, no
-    // multi-destination rows: both candidates per lane in one pass
-    if (n_cur <= 64 && (!use_pair || pair.n <= 64) && !__ballot((e0 >> 24) >= 2)) {
+    // common case: one chunk of states, one of two-byte row entries, rows of
+    // at most two destinations (a persistent state's row is itself plus the
+    // state it advances to): every lane's candidates in one pass
+    if (n_cur <= 64 && (!use_pair || pair.n <= 64) && !__ballot((e0 >> 24) >= 3)) {
         const bool okp = use_pair && lane_id() < pair.n;
-        const uint32_t d = e0 & kIdMask;
+        const uint32_t cnt = e0 >> 24;
+        uint32_t x0 = e0 & kIdMask, x1 = 0;
+        const bool two = __ballot(cnt == 2) != 0;  // a row in the store
+        if (two && cnt == 2) {
+            const uint32_t o = x0;
+            x0 = g.as_multi[o];
+            x1 = g.as_multi[o + 1];
+        }
         if constexpr (SET::kBounded) {
             const bool n1 = fut.insert_if(okp, pair.v);
-            const bool n2 = fut.insert_if((e0 >> 24) == 1, d);
-            const int n = list_append_all<CAP>(fl, 0, n1, pair.v);
-            return list_append_all<CAP>(fl, n, n2, d);
+            const bool n2 = fut.insert_if(cnt != 0, x0);
+            int n = list_append_all<CAP>(fl, 0, n1, pair.v);
+            n = list_append_all<CAP>(fl, n, n2, x0);
+            if (two) {
+                const bool n3 = fut.insert_if(cnt == 2, x1);
+                n = list_append_all<CAP>(fl, n, n3, x1);
+            }
+            return n;
         } else {
             const bool n1 = okp && fut.insert(pair.v);
-            const bool n2 = (e0 >> 24) == 1 && fut.insert(d);
-            const int n = list_append<CAP>(fl, 0, n1, pair.v);
-            return list_append<CAP>(fl, n, n2, d);
+            const bool n2 = cnt != 0 && fut.insert(x0);
+            int n = list_append<CAP>(fl, 0, n1, pair.v);
+            n = list_append<CAP>(fl, n, n2, x0);
+            if (two) {
+                const bool n3 = cnt == 2 && fut.insert(x1);
+                n = list_append<CAP>(fl, n, n3, x1);
+            }
+            return n;
         }
     }
     int n = 0;
