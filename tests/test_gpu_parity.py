@@ -132,3 +132,35 @@ def test_graph_replay_and_append(workloads):
         # interleave eager work between replays
         assert s.batched_scan([w.inputs.string(1)]) == [exp[1]]
     s.close()
+
+
+def test_large_full_size(workloads):
+    # the large workload at its full size (65,536 strings, 92 MB): the engine
+    # (hashed storage; AS and the learned table) equals the plain iNFA kernel
+    # bit for bit, every planted string reports a rule, ids are sorted and
+    # unique, and a sample holding the 64 longest strings equals the oracle
+    from admatch_amd import selector
+    w, path = workloads("large")
+    buf, off = w.inputs.buf, w.inputs.offsets
+    n = off.size - 1
+    assert n == 65536
+    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+    print("[case] large full size, plain iNFA", flush=True)
+    s.scan_device(d_buf.ptr, d_off.ptr, n, E.KIND_PLAIN, E.MODE_INFA)
+    ref = E.unpack(*s.collect(n))
+    s.set_variant(2, 256, 64, 1)
+    for mode, wt in ((E.MODE_AS, None), (E.MODE_LEARNED, selector.load_weights())):
+        s.set_selector(wt)
+        print(f"[case] large full size, hashed mode={E.MODE_NAMES[mode]} trained={wt is not None}", flush=True)
+        s.scan_device(d_buf.ptr, d_off.ptr, n, E.KIND_ENGINE, mode)
+        got = E.unpack(*s.collect(n))
+        assert got == ref, (mode, first_diff(got, ref))
+    s.close()
+    assert all(ref[i] for i in np.nonzero(w.inputs.planted)[0])
+    assert all(r == sorted(set(r)) for r in ref)
+    lens = np.diff(off)
+    sample = sorted(set(np.argsort(lens)[-64:].tolist()) | set(range(0, n, 512)))
+    sbuf, soff = E.pack_strings([w.inputs.string(i) for i in sample])
+    exp = oracle.OracleNFA(path).batch_scan(sbuf, soff, nthreads=os.cpu_count() or 4)
+    assert [ref[i] for i in sample] == exp
