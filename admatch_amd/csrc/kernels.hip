@@ -1195,9 +1195,6 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         ctx.pos = b;
         bool log_step = false;
         int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
-        // without two-byte rows the first positions of the implicit set are
-        // not reachable from the table: list them before an AS step
-        if (m == M_AS && !have_pair && imp >= 0) materialize();
         if (m == M_AS && !(sparse && can_as)) m = M_INFA;
         int n_fut = 0;
         bool fsp = false;
@@ -1224,6 +1221,9 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             n_act = n_fut;
             imp = c;
             wave_sync();
+            // without two-byte rows the next step could not reach the first
+            // positions of the implicit set from the table: list them now
+            if (!have_pair) materialize();
         } else {
             if (MODE == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
             // bitmap steps counted here, off the AS path (AS steps: the string
