@@ -341,6 +341,10 @@ int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap) {
             if (int rc = h->eng.policy_table(t.data(), err)) return fail(rc, err);
             return give(t.data(), int64_t(t.size()), 4);
         }
+        case 17: return give(g.wbs_off.data(), int64_t(g.wbs_off.size()), 4);
+        case 18: return give(g.wbs_dst.data(), int64_t(g.wbs_dst.size()), 4);
+        case 19: return give(g.wbi_off.data(), int64_t(g.wbi_off.size()), 4);
+        case 20: return give(g.wbi_dst.data(), int64_t(g.wbi_dst.size()), 4);
         default: return fail(ITA_ERR_ARG, "bad table id");
     }
 }

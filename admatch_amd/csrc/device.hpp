@@ -31,6 +31,13 @@ struct DevGraph {
     const int4* sym_stats;    // [256]: |T_c|, k_c, |leftover_c|, start in alw_dst
     int32_t wb_begin, wb_count;  // word-boundary slice of list
     int32_t wb_parallel;         // no boundary destination is a boundary source
+    // boundary successors per state and per implicit set (null unless
+    // wb_parallel; read by the LDS-storage engine): wbs_dst[wbs_off[s] ..),
+    // wbi_dst[wbi_off[r] ..), r = byte of the implicit row or 256
+    const uint32_t* wbs_off;  // [S+1]
+    const uint32_t* wbs_dst;
+    const int32_t* wbi_off;   // [258]
+    const uint32_t* wbi_dst;
 };
 
 // Per-launch arguments of the batched engine.

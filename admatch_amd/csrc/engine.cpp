@@ -192,6 +192,7 @@ int Engine::load(const char* path, std::string& err) {
     if (load_nfa(path, g, err) != 0) return ADM_ERR_LOAD;
     build_tko(g);
     build_as(g);
+    build_wb_rows(g);
     g_ = std::move(g);
     loaded_ = true;
     return ADM_OK;
@@ -299,6 +300,17 @@ int Engine::upload(std::string& err) {
     } else {
         dev_.pair_off = nullptr;
         dev_.pair_dst = nullptr;
+    }
+    if (!g_.wbs_off.empty()) {
+        ADM_PUT(g_.wbs_off, wbs_off)
+        ADM_PUT(g_.wbs_dst, wbs_dst)
+        ADM_PUT(g_.wbi_off, wbi_off)
+        ADM_PUT(g_.wbi_dst, wbi_dst)
+    } else {
+        dev_.wbs_off = nullptr;
+        dev_.wbs_dst = nullptr;
+        dev_.wbi_off = nullptr;
+        dev_.wbi_dst = nullptr;
     }
     std::vector<int4> st(256);
     for (int c = 0; c < 256; ++c)

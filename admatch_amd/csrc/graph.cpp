@@ -454,4 +454,38 @@ void build_as(Graph& g) {
     }
 }
 
+void build_wb_rows(Graph& g) {
+    g.wbs_off.clear();
+    g.wbs_dst.clear();
+    g.wbi_off.clear();
+    g.wbi_dst.clear();
+    if (g.wb_count() == 0 || !g.wb_chain_free()) return;
+    // CSR by source (the boundary list is sorted by (src, dst))
+    const int32_t b0 = g.sym_off[kWordBoundary], b1 = g.sym_off[kWordBoundary + 1];
+    g.wbs_off.assign(size_t(g.S) + 1, 0);
+    for (int32_t t = b0; t < b1; ++t) g.wbs_off[size_t(g.list[size_t(t)].src) + 1] += 1;
+    for (int32_t s = 0; s < g.S; ++s) g.wbs_off[size_t(s) + 1] += g.wbs_off[size_t(s)];
+    g.wbs_dst.resize(size_t(b1 - b0));
+    for (int32_t t = b0; t < b1; ++t) g.wbs_dst[size_t(t - b0)] = g.list[size_t(t)].dst;
+    // rows of the implicit sets
+    std::vector<uint32_t> row;
+    g.wbi_off.assign(258, 0);
+    for (int r = 0; r <= 256; ++r) {
+        g.wbi_off[size_t(r)] = int32_t(g.wbi_dst.size());
+        row.clear();
+        auto add = [&](uint32_t st) {
+            for (uint32_t k = g.wbs_off[st]; k < g.wbs_off[size_t(st) + 1]; ++k) row.push_back(g.wbs_dst[k]);
+        };
+        if (r < 256) {
+            for (int32_t k = g.alw_off[size_t(r)]; k < g.alw_off[size_t(r) + 1]; ++k) add(g.alw_dst[size_t(k)] & 0x00ffffffu);
+        } else {
+            for (uint32_t st : g.alw_states) add(st);
+        }
+        std::sort(row.begin(), row.end());
+        row.erase(std::unique(row.begin(), row.end()), row.end());
+        g.wbi_dst.insert(g.wbi_dst.end(), row.begin(), row.end());
+    }
+    g.wbi_off[257] = int32_t(g.wbi_dst.size());
+}
+
 }  // namespace admatch

@@ -86,6 +86,15 @@ struct Graph {
     // ADM_MAX_PAIR_ENTRIES); the engine then lists the first positions
     // before each AS step.
     std::vector<uint32_t> pair_off, pair_dst;
+    // Word-boundary rows, built when the boundary list is chain-free (one
+    // parallel pass equals the in-place pass): the boundary successors of
+    // each state, wbs_dst[wbs_off[s] .. wbs_off[s+1]), and of each implicit
+    // set, wbi_dst[wbi_off[r] .. wbi_off[r+1]) with r = the byte of an
+    // always-active row (the always-active states and the first positions
+    // reached on it) or r = 256 for the always-active states alone.  Empty
+    // otherwise.
+    std::vector<uint32_t> wbs_off, wbs_dst, wbi_dst;
+    std::vector<int32_t> wbi_off;
 
     // parse statistics
     int64_t n_persis_lines = 0, n_bad_lines = 0, n_raw_transitions = 0;
@@ -117,5 +126,8 @@ void build_tko(Graph& g, int32_t threshold = -1);
 
 // Sparse active-state rows from the iNFA lists, and their device form.
 void build_as(Graph& g);
+
+// Word-boundary rows (after build_as); left empty unless chain-free.
+void build_wb_rows(Graph& g);
 
 }  // namespace admatch

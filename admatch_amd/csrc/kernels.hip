@@ -776,6 +776,39 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     };
 
     auto wb_pass = [&]() {
+        if (g.wbs_off != nullptr && sparse) {
+            // chain-free list with boundary rows: the boundary successors of
+            // the listed states and of the implicit set (precomputed per
+            // implicit row), without materialising the implicit set
+            const int n0 = n_act;
+            if (imp != -1) {
+                const int r = imp >= 0 ? imp : 256;
+                const int rb = g.wbi_off[r], rn = g.wbi_off[r + 1] - rb;
+                for (int t0 = 0; t0 < rn; t0 += 64) {
+                    const int t = t0 + lane_id();
+                    const bool ok = t < rn;
+                    const uint32_t v = ok ? g.wbi_dst[rb + t] : 0u;
+                    const bool nw = ok && set_new(cur, v);
+                    n_act = list_append<CAP>(cl, n_act, nw, v);
+                }
+            }
+            for (int k0 = 0; k0 < n0; k0 += 64) {
+                const int k = k0 + lane_id();
+                const uint32_t st = k < n0 ? (cl[k] & kIdMask) : 0u;
+                const uint32_t b = k < n0 ? g.wbs_off[st] : 0u;
+                const uint32_t e = k < n0 ? g.wbs_off[st + 1] : 0u;
+                for (uint32_t j = 0; __ballot(b + j < e); ++j) {
+                    const bool ok = b + j < e;
+                    const uint32_t v = ok ? g.wbs_dst[b + j] : 0u;
+                    const bool nw = ok && set_new(cur, v);
+                    n_act = list_append<CAP>(cl, n_act, nw, v);
+                }
+            }
+            n_act = uniform(n_act);
+            if (n_act > CAP) sparse = false;
+            wave_sync();
+            return;
+        }
         materialize();
         if (g.wb_parallel) {
             // no destination is a source: one parallel pass, the wave's lanes

@@ -244,7 +244,8 @@ class Scratch:
 
     def table(self, which: int) -> np.ndarray:
         dt = {4: np.uint64, 9: np.uint64, 1: np.uint32, 6: np.uint32, 7: np.uint32, 8: np.uint32, 10: np.uint32,
-              11: np.uint32, 13: np.uint32, 14: np.uint32, 15: np.uint32, 16: np.uint32}.get(which, np.int32)
+              11: np.uint32, 13: np.uint32, 14: np.uint32, 15: np.uint32, 16: np.uint32, 17: np.uint32, 18: np.uint32,
+              20: np.uint32}.get(which, np.int32)
         n = lib().ita_table(self._h, which, None, 0)
         if n < 0:
             _err(int(n))

@@ -174,8 +174,13 @@ int ita_info(ita_handle* h, int64_t* out11);
  * 15 two-byte rows (uint32), 16 the learned selector's policy table
  * (512 uint32, after ita_set_selector: method of (bucket b, list valid v,
  * byte c) in bits 2*(c%16).. of word (2*b + v)*16 + c/16, bucket =
- * min(15, floor(log2(explicit count + 1)))).  Returns the element count;
- * copies min(count, cap) elements when out != NULL. */
+ * min(15, floor(log2(explicit count + 1)))), 17 word-boundary successors
+ * per state: offsets (S+1 uint32), 18 their destinations (uint32), 19
+ * word-boundary successors of the implicit sets: offsets (258 int32, row c
+ * for the always-active row of byte c, row 256 for the always-active states
+ * alone), 20 their destinations (uint32); 17-20 are empty unless no
+ * word-boundary destination is a word-boundary source.  Returns the element
+ * count; copies min(count, cap) elements when out != NULL. */
 int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap);
 
 /* Learned-selector training on the GPU (no counterpart in the reference):

@@ -255,3 +255,45 @@ def test_pair_row_limit(gold_dir, monkeypatch):
 
 def lib_count(s, which):
     return int(E.lib().ita_table(s._h, which, None, 0))
+
+
+def test_word_boundary_rows(tmp_path):
+    # chain-free boundary lists get per-state and per-implicit-set rows: the
+    # boundary successors of every state, and of every always-active row
+    from admatch_amd import regex, synth
+    text, _ = regex.compile_rules(synth.frontend_patterns(60, seed=5))
+    path = str(tmp_path / "wb.nfa")
+    with open(path, "w") as f:
+        f.write(text)
+    s = E.Scratch(E.AD_MATCHER, path)
+    info = s.info()
+    off, lst = s.table(0), s.table(1).reshape(-1, 2)
+    wb = lst[off[256]:off[257]]
+    assert info["wb_transitions"] == len(wb) > 0
+    assert not (set(wb[:, 0].tolist()) & set(wb[:, 1].tolist()))  # chain-free
+    succ = {}
+    for a, b in wb.tolist():
+        succ.setdefault(a, []).append(b)
+    soff, sdst = s.table(17), s.table(18)
+    assert soff.size == info["states"] + 1
+    for st in range(info["states"]):
+        assert sdst[soff[st]:soff[st + 1]].tolist() == sorted(succ.get(st, [])), st
+    ioff, idst = s.table(19), s.table(20)
+    aoff, adst = s.table(12), s.table(13)
+    alw = [int(x & 0xFFFFFF) for x in adst[aoff[0]:aoff[1]] if x >> 31]
+    for r in list(range(256)) + [256]:
+        src = alw if r == 256 else [int(x & 0xFFFFFF) for x in adst[aoff[r]:aoff[r + 1]]]
+        want = sorted(set(d for x in src for d in succ.get(x, [])))
+        assert idst[ioff[r]:ioff[r + 1]].tolist() == want, r
+    s.close()
+
+
+def test_word_boundary_rows_absent_with_chains(tmp_path):
+    # a boundary destination that is also a boundary source: no rows
+    path = str(tmp_path / "chain.nfa")
+    with open(path, "w") as f:
+        f.write("4\n0 : initial\n0 -> 0 : 0|255\n0 -> 1 : 97\n1 -> 2 : 256\n2 -> 3 : 256\n3 : accepting 1\n")
+    s = E.Scratch(E.AD_MATCHER, path)
+    assert s.info()["wb_transitions"] == 2
+    assert all(int(E.lib().ita_table(s._h, w, None, 0)) == 0 for w in (17, 18, 19, 20))
+    s.close()
