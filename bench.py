@@ -252,7 +252,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
-              "strings shard across GPUs without communication, see DESIGN.md", file=sys.stderr)
+              "strings would shard across GPUs by length-balanced shares with a replica of the rule set each, "
+              "and nothing but the host result gather crosses GPUs", file=sys.stderr)
         sys.exit(2)
 
     from admatch_amd import engine as E
