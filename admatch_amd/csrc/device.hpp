@@ -88,6 +88,11 @@ enum SelectMode : int32_t {
     MODE_LEARNED_AS = 8,     // internal: policy table whose reachable list-valid entries all pick AS
 };
 
+// Hashed-storage kernels are also built with this bit added to the mode, for
+// rule sets without word-boundary transitions: the boundary code is
+// compiled out of the step loop.
+constexpr int32_t kModeNoWb = 0x100;
+
 enum Method : int32_t { M_INFA = 0, M_TKO = 1, M_AS = 2 };
 
 constexpr int kFeatures = 16;

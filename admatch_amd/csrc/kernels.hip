@@ -1056,6 +1056,9 @@ template <int CAP, int MODE>
 __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* gA, uint64_t* gB,
                               uint32_t* hA, uint32_t* hB, uint32_t* listA, uint32_t* listB, int* msteps) {
     using HS = HashSet<2 * CAP>;
+    // the selector mode, and whether boundary code is compiled in
+    constexpr int M = MODE & ~kModeNoWb;
+    constexpr bool kWb = (MODE & kModeNoWb) == 0;
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int L = int(a.offsets[sid + 1] - beg);
@@ -1077,7 +1080,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     wave_sync();
     ByteWindow win{a.text + beg, L, 0, 0u};
     if (L > 0) win.load(0);
-    const bool has_wb = g.wb_count > 0;
+    const bool has_wb = kWb && g.wb_count > 0;
     const bool have_pair = g.pair_off != nullptr;
     int imp = -2;  // implicitly held states, as in scan_string
     auto alw_range = [&](int cc, int& b0, int& n0) {
@@ -1227,7 +1230,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         ctx.sparse = sparse;
         ctx.pos = b;
         bool log_step = false;
-        int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
+        int m = choose_method<M>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
         if (m == M_AS && !(sparse && can_as)) m = M_INFA;
         int n_fut = 0;
         bool fsp = false;
@@ -1258,7 +1261,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             // positions of the implicit set from the table: list them now
             if (!have_pair) materialize();
         } else {
-            if (MODE == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
+            if (M == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
             // bitmap steps counted here, off the AS path (AS steps: the string
             // length minus these)
             msteps[0] += m == M_TKO ? 0 : 1;
@@ -1544,6 +1547,10 @@ hipError_t engine_occ_m(size_t lds, int* blocks) {
             case MODE_AS: return FN<WPB, CAP, MODE_AS, true>(__VA_ARGS__);              \
             case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, true>(__VA_ARGS__);    \
             case MODE_LEARNED_AS: return FN<WPB, CAP, MODE_LEARNED_AS, true>(__VA_ARGS__); \
+            case MODE_AS | kModeNoWb: return FN<WPB, CAP, MODE_AS | kModeNoWb, true>(__VA_ARGS__); \
+            case MODE_LEARNED | kModeNoWb: return FN<WPB, CAP, MODE_LEARNED | kModeNoWb, true>(__VA_ARGS__); \
+            case MODE_LEARNED_AS | kModeNoWb:                                           \
+                return FN<WPB, CAP, MODE_LEARNED_AS | kModeNoWb, true>(__VA_ARGS__);    \
             default: return hipErrorInvalidValue;                                       \
         }                                                                               \
     }                                                                                   \
