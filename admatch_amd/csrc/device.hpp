@@ -89,9 +89,10 @@ enum SelectMode : int32_t {
 };
 
 // Hashed-storage kernels are also built with this bit added to the mode, for
-// rule sets without word-boundary transitions: the boundary code is
-// compiled out of the step loop.
-constexpr int32_t kModeNoWb = 0x100;
+// rule sets without word-boundary transitions and with two-byte rows: the
+// boundary code and the path for missing rows are compiled out of the step
+// loop.
+constexpr int32_t kModeLean = 0x100;
 
 enum Method : int32_t { M_INFA = 0, M_TKO = 1, M_AS = 2 };
 

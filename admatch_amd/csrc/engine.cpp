@@ -391,8 +391,8 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     }
     ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
     const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
-    // hashed kernels without the boundary code for rule sets without boundaries
-    const int32_t kmode = (hs && g_.wb_count() == 0) ? (mode | kModeNoWb) : mode;
+    // lean hashed kernels for rule sets without boundaries and with two-byte rows
+    const int32_t kmode = (hs && g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
     ADM_HIP(launch_engine(var_.wpb, var_.cap, kmode, hs, grid, lds, st, dev_, a));
     return ADM_OK;
 }
@@ -443,7 +443,7 @@ int Engine::engine_grid(int32_t mode, int32_t n, bool hs, int* grid, std::string
         return ADM_ERR_TOO_LARGE;
     }
     int blocks = 0;
-    const int32_t kmode = (hs && g_.wb_count() == 0) ? (mode | kModeNoWb) : mode;
+    const int32_t kmode = (hs && g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
     ADM_HIP(engine_occupancy(var_.wpb, var_.cap, kmode, hs, lds, &blocks));
     if (blocks <= 0) {
         err = "engine variant does not fit on a CU";

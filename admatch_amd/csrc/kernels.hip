@@ -1056,9 +1056,10 @@ template <int CAP, int MODE>
 __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* gA, uint64_t* gB,
                               uint32_t* hA, uint32_t* hB, uint32_t* listA, uint32_t* listB, int* msteps) {
     using HS = HashSet<2 * CAP>;
-    // the selector mode, and whether boundary code is compiled in
-    constexpr int M = MODE & ~kModeNoWb;
-    constexpr bool kWb = (MODE & kModeNoWb) == 0;
+    // the selector mode, and whether the boundary code and the path for
+    // missing two-byte rows are compiled in
+    constexpr int M = MODE & ~kModeLean;
+    constexpr bool kLean = (MODE & kModeLean) != 0;
     const int W = g.W;
     const int64_t beg = a.offsets[sid];
     const int L = int(a.offsets[sid + 1] - beg);
@@ -1080,8 +1081,8 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     wave_sync();
     ByteWindow win{a.text + beg, L, 0, 0u};
     if (L > 0) win.load(0);
-    const bool has_wb = kWb && g.wb_count > 0;
-    const bool have_pair = g.pair_off != nullptr;
+    const bool has_wb = !kLean && g.wb_count > 0;
+    const bool have_pair = kLean || g.pair_off != nullptr;
     int imp = -2;  // implicitly held states, as in scan_string
     auto alw_range = [&](int cc, int& b0, int& n0) {
         b0 = g.sym_stats[cc].w;
@@ -1547,10 +1548,10 @@ hipError_t engine_occ_m(size_t lds, int* blocks) {
             case MODE_AS: return FN<WPB, CAP, MODE_AS, true>(__VA_ARGS__);              \
             case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, true>(__VA_ARGS__);    \
             case MODE_LEARNED_AS: return FN<WPB, CAP, MODE_LEARNED_AS, true>(__VA_ARGS__); \
-            case MODE_AS | kModeNoWb: return FN<WPB, CAP, MODE_AS | kModeNoWb, true>(__VA_ARGS__); \
-            case MODE_LEARNED | kModeNoWb: return FN<WPB, CAP, MODE_LEARNED | kModeNoWb, true>(__VA_ARGS__); \
-            case MODE_LEARNED_AS | kModeNoWb:                                           \
-                return FN<WPB, CAP, MODE_LEARNED_AS | kModeNoWb, true>(__VA_ARGS__);    \
+            case MODE_AS | kModeLean: return FN<WPB, CAP, MODE_AS | kModeLean, true>(__VA_ARGS__); \
+            case MODE_LEARNED | kModeLean: return FN<WPB, CAP, MODE_LEARNED | kModeLean, true>(__VA_ARGS__); \
+            case MODE_LEARNED_AS | kModeLean:                                           \
+                return FN<WPB, CAP, MODE_LEARNED_AS | kModeLean, true>(__VA_ARGS__);    \
             default: return hipErrorInvalidValue;                                       \
         }                                                                               \
     }                                                                                   \
