@@ -454,9 +454,19 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
         const bool nw = cnt == 1 && fut.insert(d);
         n = list_append<CAP>(fl, n, nw, d);
         if (SET::kBounded && n > CAP) return n;
-        // short rows: each lane walks its own row, four loads in flight
+        // short rows: each lane walks its own row, two or four loads in flight
         const int len = (cnt >= 2 && cnt < 255) ? int(cnt) : 0;
-        if (__ballot(len > 0)) {
+        if (__ballot(len > 0) && !__ballot(len > 2)) {
+            // rows of two (a persistent state and the state it advances to)
+            const uint32_t y0 = len ? g.as_multi[d] : 0u;
+            const uint32_t y1 = len ? g.as_multi[d + 1] : 0u;
+            const bool n0 = len && fut.insert(y0);
+            n = list_append<CAP>(fl, n, n0, y0);
+            if (SET::kBounded && n > CAP) return n;
+            const bool n1 = len && fut.insert(y1);
+            n = list_append<CAP>(fl, n, n1, y1);
+            if (SET::kBounded && n > CAP) return n;
+        } else if (__ballot(len > 0)) {
             uint32_t x[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) x[j] = j < len ? g.as_multi[d + j] : 0u;
