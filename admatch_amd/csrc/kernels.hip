@@ -78,7 +78,9 @@ struct HashSet {
     // insert for every lane, d only where pred: true if d was new
     __device__ __forceinline__ bool insert_if(bool pred, uint32_t d) const {
         const uint32_t key = pred ? d : kSink;
-        uint32_t i = pred ? slot(d) : uint32_t(H + lane_id());
+        // the home slot for every lane, selected after (no lane masking)
+        const uint32_t h = slot(d);
+        uint32_t i = pred ? h : uint32_t(H + lane_id());
         uint32_t old = atomicCAS(&tab[i], kEmpty, key);
         while (old != kEmpty && old != key) {
             i = (i + 1) & uint32_t(H - 1);
