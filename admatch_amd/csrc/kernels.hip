@@ -1222,12 +1222,11 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     int cached = -1;
     for (int b = 0; b < L; ++b) {
         // the next byte is read one step ahead (two-byte rows, word boundaries)
+        // (at the last byte the window reads 0 past the end and the row is unused)
         const int c = c_next;
         RowSlice pair_next{0u, 0, 0};
-        if (b + 1 < L) {
-            c_next = win.at(b + 1);
-            if (have_pair) pair_next = load_pair(c, c_next);
-        }
+        c_next = win.at(b + 1);
+        if (have_pair) pair_next = load_pair(c, c_next);
         auto total_active = [&]() {
             int n_tot = n_act;
             if (imp >= 0) {
