@@ -1206,7 +1206,9 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         RowSlice r;
         // read-only offsets: scalar loads
         using CU32 = const __attribute__((address_space(4))) uint32_t;
-        const CU32* o = (CU32*)(g.pair_off) + (c1 * 256 + c2);
+        // a 32-bit byte offset lets the load take it as its scalar offset
+        const uint32_t off = ((uint32_t(c1) << 8) | uint32_t(c2)) << 2;
+        const CU32* o = (const CU32*)((const __attribute__((address_space(4))) char*)(g.pair_off) + off);
         r.b = int(o[0]);
         r.n = int(o[1]) - r.b;
         r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
