@@ -246,7 +246,7 @@ int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3) {
 int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs, int32_t store) {
     if (int rc = check(h)) return rc;
     if ((wpb != 1 && wpb != 2 && wpb != 4) || (cap != 128 && cap != 256 && cap != 512 && cap != 1024) ||
-        (plain_bs != 64 && plain_bs != 128 && plain_bs != 256) || (store != 0 && store != 1))
+        (plain_bs != 64 && plain_bs != 128 && plain_bs != 256 && plain_bs != 512 && plain_bs != 1024) || (store != 0 && store != 1))
         return fail(ITA_ERR_ARG, "unsupported variant");
     admatch::Variant v;
     v.wpb = wpb;

@@ -1,222 +1,17 @@
-// Scan kernels for gfx950 (wave64, 160 KB LDS per CU).
+// Kernels of the scan path besides the engine's step loop (whose kernels are
+// instantiated in engine_c*_*.hip from engine_kernels.hpp):
 //
 //  * plain_infa_kernel: the single-kernel iNFA path -- one workgroup per
 //    string, state bit vectors in LDS, every step scatters the byte's whole
 //    transition list (the design of src/iNFA_kernel.cu:12-173, re-laid out
 //    for u64 words and LDS ds_or atomics).  It is the throughput baseline.
-//  * engine_kernel: the adaptive engine -- a persistent grid whose waves pull
-//    strings longest-first from a work queue; one wave owns one string, so a
-//    step needs no workgroup barrier.  Each step runs one of three methods,
-//    chosen per step by the selector:
-//      iNFA  scatter over the byte's (src,dst) list      (src/iNFA_kernel.cu:89-132)
-//      TKO   top-k offset groups as word shifts + leftover scatter
-//                                                        (src/TKO_kernel.cu:89-188)
-//      AS    destination rows of the active states       (src/AS_kernel.cu:74-125)
-//    The active set is kept both as a bitmap and, while it has at most LCAP
-//    states, as a list, so the AS and iNFA methods never touch all W words.
 //  * bin kernels: longest-first processing order by length bins.
-//  * learned selector: the MLP is evaluated once per (byte, active-count
-//    bucket, list validity) into a 2-bit policy table (policy_kernel); a
-//    step reads its method with one scalar load.
-#include "device.hpp"
+//  * policy_kernel: the learned selector's network evaluated once per (byte,
+//    active-count bucket, list validity) into a 2-bit table.
+#include "engine_kernels.hpp"
+#include "launch.hpp"
 
 namespace admatch {
-
-// ------------------------------------------------------------------ common
-// List entries carry bit 31 for always-active states (their rows come from
-// the per-byte always-active list instead of the AS table).
-constexpr uint32_t kTag = 0x80000000u;
-constexpr uint32_t kIdMask = 0x00ffffffu;
-
-__device__ __forceinline__ bool set_new(uint64_t* bits, uint32_t d) {
-    const unsigned long long m = 1ull << (d & 63);
-    const unsigned long long old = atomicOr(reinterpret_cast<unsigned long long*>(&bits[d >> 6]), m);
-    return !(old & m);
-}
-
-__device__ __forceinline__ void set_bit_atomic(uint64_t* bits, uint32_t d) {
-    atomicOr(reinterpret_cast<unsigned long long*>(&bits[d >> 6]), 1ull << (d & 63));
-}
-
-// Destination sets of a step.  BitSet: a bitmap of all S states.  HashSet: an
-// open-addressing table of H slots in LDS (linear probing, kEmpty marks a free
-// slot); it holds at most CAP + 64 keys, so steps that use it stop early
-// (kBounded) once the list passes CAP and are redone on the bitmaps.
-constexpr uint32_t kEmpty = 0xffffffffu;
-
-struct BitSet {
-    static constexpr bool kBounded = false;
-    uint64_t* bits;
-    __device__ __forceinline__ bool insert(uint32_t d) const { return set_new(bits, d); }
-};
-
-// Each table is followed by 64 sink slots, one per lane, that permanently
-// hold kSink: a lane with nothing to insert compare-and-swaps its sink slot,
-// so inserts need no lane masking.
-constexpr uint32_t kSink = 0xfffffffeu;
-
-template <int H>
-struct HashSet {
-    static_assert((H & (H - 1)) == 0, "table size must be a power of two");
-    static constexpr bool kBounded = true;
-    static constexpr int kSlots = H + 64;  // u32 per table including the sink slots
-    uint32_t* tab;
-    __device__ __forceinline__ static uint32_t slot(uint32_t d) {
-        return (d * 2654435761u) >> (32 - __builtin_ctz(uint32_t(H)));
-    }
-    // true if d was not in the set.  One compare-and-swap on the home slot;
-    // the probe loop runs only after a collision with another key.
-    __device__ __forceinline__ bool insert(uint32_t d) const {
-        uint32_t i = slot(d);
-        uint32_t old = atomicCAS(&tab[i], kEmpty, d);
-        while (old != kEmpty && old != d) {
-            i = (i + 1) & uint32_t(H - 1);
-            old = atomicCAS(&tab[i], kEmpty, d);
-        }
-        return old == kEmpty;
-    }
-    // insert for every lane, d only where pred: true if d was new
-    __device__ __forceinline__ bool insert_if(bool pred, uint32_t d) const {
-        const uint32_t key = pred ? d : kSink;
-        // the home slot for every lane, selected after (no lane masking)
-        const uint32_t h = slot(d);
-        uint32_t i = pred ? h : uint32_t(H + lane_id());
-        uint32_t old = atomicCAS(&tab[i], kEmpty, key);
-        while (old != kEmpty && old != key) {
-            i = (i + 1) & uint32_t(H - 1);
-            old = atomicCAS(&tab[i], kEmpty, key);
-        }
-        return old == kEmpty;
-    }
-    // the sink slots (once per table)
-    __device__ __forceinline__ void init_sink() const { tab[H + lane_id()] = kSink; }
-    __device__ __forceinline__ bool contains(uint32_t d) const {
-        uint32_t i = slot(d);
-        for (;;) {
-            const uint32_t v = tab[i];
-            if (v == d) return true;
-            if (v == kEmpty) return false;
-            i = (i + 1) & uint32_t(H - 1);
-        }
-    }
-    // every slot free again (whole wave)
-    __device__ __forceinline__ void clear() const {
-        static_assert(H % 256 == 0, "the wave clears 256 slots per store");
-        uint4* t4 = reinterpret_cast<uint4*>(tab);
-#pragma unroll
-        for (int r = 0; r < H / 256; ++r) t4[r * 64 + lane_id()] = make_uint4(kEmpty, kEmpty, kEmpty, kEmpty);
-    }
-};
-
-// Byte i of a string, read through a 256-byte window held one u32 per lane
-// (strings shorter than 2^31 bytes).
-struct ByteWindow {
-    const uint8_t* p;
-    int len;
-    int base;    // first byte index of the window
-    uint32_t w;  // lane l holds bytes base+4l .. base+4l+3
-
-    __device__ void load(int b) {
-        base = b;
-        const int i = b + 4 * lane_id();
-        uint32_t v = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (i + k < len) v |= uint32_t(p[i + k]) << (8 * k);
-        w = v;
-    }
-    __device__ int at(int i) {  // i uniform, base <= i
-        if (uint32_t(i - base) >= 256u) load(i);
-        const int off = i - base;
-        const uint32_t v = uint32_t(__builtin_amdgcn_readlane(int(w), off >> 2));
-        return int((v >> (8 * (off & 3))) & 0xffu);
-    }
-};
-
-// Appends v for every lane with pred; returns the new (uniform) count,
-// which may exceed cap (the list is then incomplete).
-template <int CAP>
-__device__ __forceinline__ int list_append(uint32_t* list, int n, bool pred, uint32_t v) {
-    const uint64_t m = __ballot(pred);
-    if (pred) {
-        const int idx = n + int(mbcnt(m));
-        if (unsigned(idx) < unsigned(CAP)) list[idx] = v;
-    }
-    return n + __popcll(m);
-}
-
-// list_append for lists followed by 64 sink entries: every lane stores (the
-// ones without a new entry into their sink entry), so no lane masking.
-template <int CAP>
-__device__ __forceinline__ int list_append_all(uint32_t* list, int n, bool pred, uint32_t v) {
-    const uint64_t m = __ballot(pred);
-    const uint32_t idx = uint32_t(n) + mbcnt(m);
-    list[(pred && idx < uint32_t(CAP)) ? idx : uint32_t(CAP + lane_id())] = v;
-    return n + __popcll(m);
-}
-
-// Ascending ids of the set bits of bits[0..W) into list; returns the count.
-template <int CAP>
-__device__ int rebuild_list(const uint64_t* bits, int W, uint32_t* list) {
-    int n = 0;
-    for (int j0 = 0; j0 < W; j0 += 64) {
-        const int j = j0 + lane_id();
-        uint64_t v = j < W ? bits[j] : 0ull;
-        const int cnt = __popcll(v);
-        const int incl = wave_scan(cnt);
-        const int tot = uniform(__shfl(incl, 63, 64));
-        if (n + tot <= CAP) {
-            int pos = n + incl - cnt;
-            while (v) {
-                const int b = __ffsll(static_cast<unsigned long long>(v)) - 1;
-                v &= v - 1;
-                list[pos++] = uint32_t(j * 64 + b);
-            }
-        }
-        n += tot;
-    }
-    return n;
-}
-
-// Writes the accepted states (cur & accept, ascending) of string sid.
-__device__ void emit_accepts(const DevGraph& g, const ScanArgs& a, int sid, const uint64_t* cur) {
-    int total = 0;
-    for (int j0 = 0; j0 < g.W; j0 += 64) {
-        const int j = j0 + lane_id();
-        const uint64_t v = j < g.W ? (cur[j] & g.accept[j]) : 0ull;
-        total += wave_sum(__popcll(v));
-    }
-    total = uniform(total);
-    int64_t start = 0;
-    if (total > 0) {
-        unsigned long long s0 = 0;
-        if (lane_id() == 0) s0 = atomicAdd(a.pool_top, (unsigned long long)total);
-        start = int64_t(__shfl(static_cast<long long>(s0), 0, 64));
-        if (start + total > a.pool_cap) {
-            if (lane_id() == 0) atomicOr(a.overflow, 1);
-            total = -total;  // marks a truncated result
-        } else {
-            int n = 0;
-            for (int j0 = 0; j0 < g.W; j0 += 64) {
-                const int j = j0 + lane_id();
-                uint64_t v = j < g.W ? (cur[j] & g.accept[j]) : 0ull;
-                const int cnt = __popcll(v);
-                const int incl = wave_scan(cnt);
-                int64_t pos = start + n + incl - cnt;
-                while (v) {
-                    const int b = __ffsll(static_cast<unsigned long long>(v)) - 1;
-                    v &= v - 1;
-                    a.res_pool[pos++] = uint32_t(j * 64 + b);
-                }
-                n += uniform(__shfl(incl, 63, 64));
-            }
-        }
-    }
-    if (lane_id() == 0) {
-        a.res_count[sid] = total;
-        a.res_start[sid] = start;
-    }
-}
 
 // ------------------------------------------------------------ plain iNFA
 // One workgroup of BS threads per string; cur/fut/persistent vectors and the
@@ -280,320 +75,6 @@ __global__ __launch_bounds__(BS) void plain_infa_kernel(DevGraph g, ScanArgs a) 
     if (tid < 64) emit_accepts(g, a, sid, cur);
 }
 
-// ------------------------------------------------------------ engine steps
-// Each helper is executed by one whole wave; counts are wave-uniform.
-
-// fut |= cur & persistent, via the active list (fut empty on entry).
-template <int CAP>
-__device__ int persist_sparse(const DevGraph& g, const uint32_t* cl, int n_cur, uint64_t* fut, uint32_t* fl) {
-    int n = 0;
-    for (int k0 = 0; k0 < n_cur; k0 += 64) {
-        const int k = k0 + lane_id();
-        const bool ok = k < n_cur;
-        const uint32_t s = ok ? (cl[k] & kIdMask) : 0u;
-        const bool p = ok && test_bit(g.persis, s);
-        if (p) set_bit_atomic(fut, s);
-        n = list_append<CAP>(fl, n, p, s);
-    }
-    return n;
-}
-
-// Scatter step over (src,dst) edges: fut |= {dst : src in cur}.
-template <int CAP, bool TRACK>
-__device__ int scatter(const uint2* edges, int cnt, const uint64_t* cur, uint64_t* fut, uint32_t* fl, int n) {
-    constexpr int U = 4;
-    for (int t0 = 0; t0 < cnt; t0 += 64 * U) {
-        uint2 e[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int t = t0 + u * 64 + lane_id();
-            e[u] = t < cnt ? edges[t] : make_uint2(0xffffffffu, 0u);
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const bool hit = e[u].x != 0xffffffffu && test_bit(cur, e[u].x);
-            if (TRACK) {
-                const bool nw = hit && set_new(fut, e[u].y);
-                n = list_append<CAP>(fl, n, nw, e[u].y);
-            } else if (hit) {
-                set_bit_atomic(fut, e[u].y);
-            }
-        }
-    }
-    return n;
-}
-
-// TKO shift groups of symbol c: fut[j] |= shift(lim_i & cur, d_i).  Lane j
-// owns word j, so no atomics.  Shifts are split as d = 64q + r with
-// 0 <= r < 64 (floor division), and the r == 0 case never shifts by 64.
-__device__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64_t* fut) {
-    const int W = g.W;
-    const int kb = g.tko_koff[c], ke = g.tko_koff[c + 1];
-    for (int i = kb; i < ke; ++i) {
-        const int d = g.tko_shift[i];
-        const int q = d >> 6;  // arithmetic shift: floor(d / 64)
-        const int r = d & 63;
-        const uint64_t* lim = g.tko_lim + int64_t(i) * W;
-        for (int j = lane_id(); j < W; j += 64) {
-            const int s0 = j - q;
-            const int s1 = s0 - 1;
-            uint64_t v = 0;
-            if (s0 >= 0 && s0 < W) v = (lim[s0] & cur[s0]) << r;
-            if (r != 0 && s1 >= 0 && s1 < W) v |= (lim[s1] & cur[s1]) >> (64 - r);
-            if (v) fut[j] |= v;
-        }
-    }
-}
-
-// Flattened processing of multi-destination rows: lane l owns a row of
-// len_l destinations starting at as_multi[start_l]; each iteration every
-// lane fetches one destination (coalesced within a row).
-template <int CAP, typename SET>
-__device__ int multi_rows(const DevGraph& g, uint32_t start, int len, const SET& fut, uint32_t* fl, int n) {
-    const int incl = wave_scan(len);
-    const int excl = incl - len;
-    const int tot = uniform(__shfl(incl, 63, 64));
-    for (int t0 = 0; t0 < tot; t0 += 64) {
-        const int t = t0 + lane_id();
-        int lo = 0, hi = 63;  // owner lane: smallest r with incl[r] > t
-#pragma unroll
-        for (int it = 0; it < 6; ++it) {
-            const int mid = (lo + hi) >> 1;
-            const int v = __shfl(incl, mid, 64);
-            if (v > t) hi = mid;
-            else lo = mid + 1;
-        }
-        const uint32_t so = uint32_t(__shfl(int(start), lo, 64));
-        const int exo = __shfl(excl, lo, 64);
-        const bool ok = t < tot;
-        const uint32_t d = ok ? g.as_multi[so + uint32_t(t - exo)] : 0u;
-        const bool nw = ok && fut.insert(d);
-        n = list_append<CAP>(fl, n, nw, d);
-        if (SET::kBounded && n > CAP) return n;
-    }
-    return n;
-}
-
-// Prefetched slice of a precomputed row list: the first 64 entries, one per
-// lane, plus where the list lives.
-struct RowSlice {
-    uint32_t v;  // entry lane_id() (valid when lane_id() < n)
-    int b, n;    // start and length in the backing array
-};
-
-// AS step (src/AS_kernel.cu:74-125 with sparse rows): fut = union of the
-// rows of the active states on byte c.  Rows of persistent states contain
-// the state itself.  The always-active states and the first positions they
-// reached on the previous byte are held implicitly (the caller records the
-// byte): their rows on c are the precomputed two-byte rows (pair, merged
-// here when use_pair) and the always-active rows of c, which become the
-// next step's implicit set.  Always-active states (bit 31) are therefore
-// neither gathered nor stored.
-template <int CAP, typename SET>
-__device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_pair, const uint32_t* cl, int n_cur,
-                       const SET& fut, uint32_t* fl) {
-    // state-major table: entry (s, c) at s*256 + c
-    const uint32_t* tab = g.as_tab + c;
-    // the first chunk's table gathers are in flight while the two-byte rows
-    // are merged
-    const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
-    const uint32_t e0 = (sv0 & kTag) ? 0u : tab[(sv0 & kIdMask) << 8];
-    // common case: one chunk of states, one of two-byte row entries, rows of
-    // at most two destinations (a persistent state's row is itself plus the
-    // state it advances to): every lane's candidates in one pass
-    if (n_cur <= 64 && (!use_pair || pair.n <= 64) && !__ballot((e0 >> 24) >= 3)) {
-        const bool okp = use_pair && lane_id() < pair.n;
-        const uint32_t cnt = e0 >> 24;
-        uint32_t x0 = e0 & kIdMask, x1 = 0;
-        const bool two = __ballot(cnt == 2) != 0;  // a row in the store
-        if (two && cnt == 2) {
-            const uint32_t o = x0;
-            x0 = g.as_multi[o];
-            x1 = g.as_multi[o + 1];
-        }
-        if constexpr (SET::kBounded) {
-            const bool n1 = fut.insert_if(okp, pair.v);
-            const bool n2 = fut.insert_if(cnt != 0, x0);
-            int n = list_append_all<CAP>(fl, 0, n1, pair.v);
-            n = list_append_all<CAP>(fl, n, n2, x0);
-            if (two) {
-                const bool n3 = fut.insert_if(cnt == 2, x1);
-                n = list_append_all<CAP>(fl, n, n3, x1);
-            }
-            return n;
-        } else {
-            const bool n1 = okp && fut.insert(pair.v);
-            const bool n2 = cnt != 0 && fut.insert(x0);
-            int n = list_append<CAP>(fl, 0, n1, pair.v);
-            n = list_append<CAP>(fl, n, n2, x0);
-            if (two) {
-                const bool n3 = cnt == 2 && fut.insert(x1);
-                n = list_append<CAP>(fl, n, n3, x1);
-            }
-            return n;
-        }
-    }
-    int n = 0;
-    if (use_pair) {
-        for (int t0 = 0; t0 < pair.n; t0 += 64) {
-            const int t = t0 + lane_id();
-            const bool ok = t < pair.n;
-            const uint32_t v = t0 == 0 ? pair.v : (ok ? g.pair_dst[pair.b + t] : 0u);
-            const bool nw = ok && fut.insert(v);
-            n = list_append<CAP>(fl, n, nw, v);
-            if (SET::kBounded && n > CAP) return n;
-        }
-    }
-    for (int k0 = 0; k0 < n_cur; k0 += 64) {
-        uint32_t e = e0;
-        if (k0 > 0) {
-            const int k = k0 + lane_id();
-            const uint32_t sv = k < n_cur ? cl[k] : kTag;
-            e = (sv & kTag) ? 0u : tab[(sv & kIdMask) << 8];
-        }
-        const uint32_t cnt = e >> 24;
-        const uint32_t d = e & kIdMask;
-        const bool nw = cnt == 1 && fut.insert(d);
-        n = list_append<CAP>(fl, n, nw, d);
-        if (SET::kBounded && n > CAP) return n;
-        // short rows: each lane walks its own row, two or four loads in flight
-        const int len = (cnt >= 2 && cnt < 255) ? int(cnt) : 0;
-        if (__ballot(len > 0) && !__ballot(len > 2)) {
-            // rows of two (a persistent state and the state it advances to)
-            const uint32_t y0 = len ? g.as_multi[d] : 0u;
-            const uint32_t y1 = len ? g.as_multi[d + 1] : 0u;
-            const bool n0 = len && fut.insert(y0);
-            n = list_append<CAP>(fl, n, n0, y0);
-            if (SET::kBounded && n > CAP) return n;
-            const bool n1 = len && fut.insert(y1);
-            n = list_append<CAP>(fl, n, n1, y1);
-            if (SET::kBounded && n > CAP) return n;
-        } else if (__ballot(len > 0)) {
-            uint32_t x[4];
-#pragma unroll
-            for (int j = 0; j < 4; ++j) x[j] = j < len ? g.as_multi[d + j] : 0u;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const bool nj = j < len && fut.insert(x[j]);
-                n = list_append<CAP>(fl, n, nj, x[j]);
-                if (SET::kBounded && n > CAP) return n;
-            }
-            for (int j = 4; __ballot(j < len); ++j) {
-                const uint32_t y = j < len ? g.as_multi[d + j] : 0u;
-                const bool nj = j < len && fut.insert(y);
-                n = list_append<CAP>(fl, n, nj, y);
-                if (SET::kBounded && n > CAP) return n;
-            }
-        }
-        // long rows (255 or more destinations): flattened over the wave
-        const bool lng = cnt == 255;
-        if (__ballot(lng)) n = multi_rows<CAP>(g, d + 1, lng ? int(g.as_multi[d]) : 0, fut, fl, n);
-        if (SET::kBounded && n > CAP) return n;
-    }
-    return n;
-}
-
-// ------------------------------------------------------------ selector
-struct StepCtx {
-    int c, c_prev, n_act, sparse, T, K, Lft, last_m;
-    int pos, len;
-};
-
-__device__ __forceinline__ int heuristic_choice(const DevGraph& g, const StepCtx& s) {
-    // AdMatch rule (src/AdMatch_kernel.cu:87) extended with an iNFA cost term
-    if (s.sparse && s.K > s.n_act) return M_AS;
-    const int64_t tko_cost = int64_t(s.K) * g.W + s.Lft;
-    return int64_t(s.T) <= tko_cost ? M_INFA : M_TKO;
-}
-
-// Selector feature i (0..15) of a step; the wave holds feature l in lane l.
-__device__ __forceinline__ float feature(const DevGraph& g, const StepCtx& s, int i) {
-    switch (i) {
-        case 0: return __log2f(1.f + float(s.n_act));
-        case 1: return 100.f * float(s.n_act) / float(g.S);
-        case 2: return __log2f(1.f + float(s.T));
-        case 3: return float(s.K);
-        case 4: return __log2f(1.f + float(s.Lft));
-        case 5: return __log2f(float(g.W));
-        case 6: return word_char(uint32_t(s.c)) ? 1.f : 0.f;
-        case 7: return float(s.c >> 5);
-        case 8: return float(s.pos) / float(s.len > 0 ? s.len : 1);
-        case 9: return __log2f(1.f + float(s.len));
-        case 10: return float(s.sparse);
-        case 11: return s.last_m == M_INFA ? 1.f : 0.f;
-        case 12: return s.last_m == M_TKO ? 1.f : 0.f;
-        case 13: return s.last_m == M_AS ? 1.f : 0.f;
-        case 14: return __log2f(1.f + float(s.len - s.pos));
-        case 15: return __log2f(1.f + float(s.K) * float(g.W));
-        default: return 0.f;
-    }
-}
-
-// 16 -> 64 -> 64 -> 3 MLP evaluated by one wave: lane j is hidden unit j;
-// fl holds feature l in lane l.  W1 is stored input-major [16][64], W2 as
-// [64 in][64 out], so every weight load is one coalesced 256-byte row.
-// Returns argmin of the three predicted log step costs among the allowed
-// methods.
-__device__ int mlp_choice(const float* w, float fl, bool allow_as) {
-    const int j = lane_id();
-    float h = w[kMlpB1 + j];
-#pragma unroll 4
-    for (int i = 0; i < kFeatures; ++i)
-        h += w[kMlpW1 + i * kHidden + j] * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fl), i));
-    h = fmaxf(h, 0.f);
-    float h2 = w[kMlpB2 + j];
-#pragma unroll 8
-    for (int k = 0; k < kHidden; ++k) {
-        const float hk = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(h), k));
-        h2 += w[kMlpW2 + k * kHidden + j] * hk;
-    }
-    h2 = fmaxf(h2, 0.f);
-    float best = 3.0e38f;
-    int arg = M_INFA;
-#pragma unroll
-    for (int m = 0; m < kOutputs; ++m) {
-        const float o = wave_sum_f(w[kMlpW3 + m * kHidden + j] * h2) + w[kMlpB3 + m];
-        const float ou = __int_as_float(uniform(__float_as_int(o)));
-        if ((m != M_AS || allow_as) && ou < best) {
-            best = ou;
-            arg = m;
-        }
-    }
-    return arg;
-}
-
-// Learned policy table: entry (bucket, sparse, c) is the method the network
-// picks for a step on byte c with an explicit active count n in bucket
-// min(15, floor(log2(n + 1))) and list validity `sparse`; 2 bits per entry,
-// 16 entries per u32, [kPolicyBuckets][2][16] words.
-constexpr int kPolicyBuckets = 16;
-constexpr int kPolicyWords = kPolicyBuckets * 2 * 16;
-static_assert(kPolicyWords == kPolicyTableWords, "policy table size");
-// Step context at which the network is evaluated for a table entry: the
-// bucket's middle count plus the mean always-active row, a step in the
-// middle of a kPolicyLen-byte string, and the method a step of that list
-// validity usually follows.
-constexpr int kPolicyLen = 1024;
-
-__device__ __forceinline__ int policy_bucket(int n_act) { return min(kPolicyBuckets - 1, 31 - __clz(n_act + 1)); }
-
-__device__ __forceinline__ StepCtx policy_ctx(const DevGraph& g, int c, int bucket, bool sparse) {
-    StepCtx s;
-    s.c = c;
-    s.c_prev = -1;
-    s.n_act = (bucket == 0 ? 0 : (3 << (bucket - 1)) - 1) + g.alw_total / 256;
-    s.sparse = sparse;
-    const int4 st = g.sym_stats[c];
-    s.T = st.x;
-    s.K = st.y;
-    s.Lft = st.z;
-    s.last_m = sparse ? M_AS : M_INFA;
-    s.pos = kPolicyLen / 2;
-    s.len = kPolicyLen;
-    return s;
-}
-
-// One wave per table entry; table zeroed beforehand.
 __global__ __launch_bounds__(64) void policy_kernel(DevGraph g, const float* mlp, uint32_t* table) {
     const int e = blockIdx.x;  // (bucket * 2 + sparse) * 256 + c
     const int c = e & 255;
@@ -602,874 +83,6 @@ __global__ __launch_bounds__(64) void policy_kernel(DevGraph g, const float* mlp
     const StepCtx s = policy_ctx(g, c, bucket, sp != 0);
     const int m = mlp_choice(mlp, feature(g, s, lane_id()), sp != 0);
     if (lane_id() == 0) atomicOr(&table[(e >> 4)], uint32_t(m) << ((c & 15) * 2));
-}
-
-// The policy table's method for a step on a bitmap-only set of n_act states.
-__device__ __forceinline__ int dense_policy(const ScanArgs& a, int n_act, int c) {
-    if (a.policy_dense >= 0) return a.policy_dense;
-    using CU32 = const __attribute__((address_space(4))) uint32_t;
-    const CU32* pt = (CU32*)(a.policy);
-    const uint32_t wv = pt[uint32_t(policy_bucket(n_act) * 2 * 16 + (c >> 4))];
-    return int((wv >> ((c & 15) * 2)) & 3u);
-}
-
-// Method for the step on ctx.c.  total_active() counts the active states
-// including the implicitly held ones (selector features).  cached: the last
-// network decision of the every-k-steps mode.
-template <int MODE, typename TOTAL>
-__device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& a, int sid, StepCtx& ctx, int n_act,
-                                             bool sparse, const TOTAL& total_active, int& cached, bool& log_step) {
-    const int c = ctx.c;
-    const int b = ctx.pos;
-    // per-byte statistics, read only by the selectors that use them
-    auto load_stats = [&]() {
-        const int4 st = g.sym_stats[c];
-        ctx.T = st.x;
-        ctx.K = st.y;
-        ctx.Lft = st.z;
-    };
-    int m;
-    switch (MODE) {
-        case MODE_INFA: m = M_INFA; break;
-        case MODE_TKO: m = M_TKO; break;
-        case MODE_AS: m = sparse ? M_AS : M_INFA; break;
-        case MODE_ADMATCH:
-            load_stats();
-            ctx.n_act = total_active();
-            m = (sparse && ctx.K > ctx.n_act) ? M_AS : M_TKO;
-            break;
-        case MODE_LEARNED: {
-            // policy table (set whenever this mode is launched): one scalar
-            // load, or none when the table holds one method for every entry
-            // of this list validity
-            m = sparse ? a.policy_sparse : a.policy_dense;
-            if (m < 0) {
-                using CU32 = const __attribute__((address_space(4))) uint32_t;
-                const CU32* pt = (CU32*)(a.policy);
-                const uint32_t wv = pt[uint32_t((policy_bucket(n_act) * 2 + int(sparse)) * 16 + (c >> 4))];
-                m = int((wv >> ((c & 15) * 2)) & 3u);
-            }
-            break;
-        }
-        case MODE_LEARNED_AS:
-            // the table picks AS whenever the list is valid; a step on a
-            // bitmap-only set reads it in the dense path (dense_policy)
-            m = sparse ? M_AS : M_INFA;
-            break;
-        case MODE_LEARNED_EVERY:
-            // the network itself, re-evaluated every mlp_every steps
-            load_stats();
-            ctx.n_act = total_active();
-            if (cached < 0 || (b % a.mlp_every) == 0) cached = mlp_choice(a.mlp, feature(g, ctx, lane_id()), true);
-            m = cached;
-            if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
-            break;
-        case MODE_LOG:
-            load_stats();
-            ctx.n_act = total_active();
-            m = heuristic_choice(g, ctx);
-            log_step = ((uint64_t(sid) * 2654435761ull + uint64_t(b)) % uint64_t(a.log_stride)) == 0;
-            break;
-        default:
-            load_stats();
-            ctx.n_act = total_active();
-            m = heuristic_choice(g, ctx);
-            break;
-    }
-    return m;
-}
-
-// ------------------------------------------------------------ engine
-
-template <int CAP>
-struct Lane {
-    // u64 words per wave: two bitmaps, two lists
-    static constexpr size_t words(int W) { return size_t(2) * W + CAP; }
-};
-
-// Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
-// Returns the fut count (list valid iff *fut_sparse).
-template <int CAP>
-__device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, const uint32_t* cl, int n_cur,
-                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse) {
-    int n = 0;
-    const int W = g.W;
-    if (m == M_TKO) {
-        tko_groups(g, c, cur, fut);
-        wave_sync();
-        scatter<CAP, false>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, fl, 0);
-        *fut_sparse = false;
-        return -1;
-    }
-    if (sparse) {
-        n = persist_sparse<CAP>(g, cl, n_cur, fut, fl);
-    } else {
-        for (int j = lane_id(); j < W; j += 64) fut[j] = cur[j] & g.persis[j];
-    }
-    wave_sync();
-    if (sparse) {
-        n = scatter<CAP, true>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, n);
-    } else {
-        scatter<CAP, false>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, 0);
-        n = -1;
-    }
-    *fut_sparse = sparse && n <= CAP;
-    return n;
-}
-
-template <int CAP>
-__device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list, int n, bool sparse, int W) {
-    if (sparse) {
-        for (int k = lane_id(); k < n; k += 64) bits[(list[k] & kIdMask) >> 6] = 0ull;
-    } else {
-        for (int j = lane_id(); j < W; j += 64) bits[j] = 0ull;
-    }
-}
-
-template <int CAP, int MODE>
-__device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
-                            uint32_t* listA, uint32_t* listB, int* msteps) {
-    const int W = g.W;
-    const int64_t beg = a.offsets[sid];
-    const int L = int(a.offsets[sid + 1] - beg);
-    msteps[3] += L;
-    uint64_t* cur = bitsA;
-    uint64_t* fut = bitsB;
-    uint32_t* cl = listA;
-    uint32_t* fl = listB;
-    // streaming: the string continues a stream whose state and last byte
-    // class were saved by the previous chunk
-    const int slot = a.stream_ids != nullptr ? a.stream_ids[sid] : -1;
-    const uint32_t sflags = slot >= 0 ? uint32_t(a.stream_flags[slot]) : 0u;
-    const uint64_t* saved = slot >= 0 ? a.stream_state + size_t(slot) * W : nullptr;
-    // invariant: both bitmaps are zero here.  The always-active states
-    // (initial and persistent) start implicit; the other initial states are
-    // listed.  A continued stream starts from its saved set, all listed.
-    const uint64_t* start = (sflags & 1u) ? saved : g.init_explicit;
-    for (int j = lane_id(); j < W; j += 64) cur[j] = start[j];
-    wave_sync();
-    int n_act = uniform(rebuild_list<CAP>(cur, W, cl));
-    bool sparse = n_act <= CAP;
-    ByteWindow win{a.text + beg, L, 0, 0u};
-    if (L > 0) win.load(0);
-    const bool has_wb = g.wb_count > 0;
-    const bool have_pair = g.pair_off != nullptr;
-    // States active but held implicitly (in neither the bitmap nor the list):
-    // imp >= 0: the always-active row of byte imp (the always-active states
-    // and the first positions they reached on that byte); imp == -2: the
-    // always-active states alone; imp == -1: none.  Set by AS steps;
-    // everything that needs the full set (other methods, the word-boundary
-    // pass, acceptance) first calls materialize().
-    int imp = (sflags & 1u) ? -1 : -2;
-    auto alw_range = [&](int cc, int& b0, int& n0) {
-        b0 = g.sym_stats[cc].w;
-        n0 = (cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total) - b0;
-    };
-    auto materialize = [&]() {
-        if (imp == -1) return;
-        int ab = 0, an = g.n_alw_states;
-        const uint32_t* src = g.alw_states;
-        if (imp >= 0) {
-            alw_range(imp, ab, an);
-            src = g.alw_dst;
-        }
-        for (int t0 = 0; t0 < an; t0 += 64) {
-            const int t = t0 + lane_id();
-            const bool ok = t < an;
-            const uint32_t v = ok ? (imp >= 0 ? src[ab + t] : (src[t] | kTag)) : 0u;
-            const bool nw = ok && set_new(cur, v & kIdMask);
-            // always-active states keep bit 31: their rows are the always-active rows
-            n_act = list_append<CAP>(cl, n_act, nw, v & (kIdMask | kTag));
-        }
-        n_act = uniform(n_act);
-        if (n_act > CAP) sparse = false;
-        imp = -1;
-        wave_sync();
-    };
-
-    auto wb_pass = [&]() {
-        if (g.wbs_off != nullptr && sparse) {
-            // chain-free list with boundary rows: the boundary successors of
-            // the listed states and of the implicit set (precomputed per
-            // implicit row), without materialising the implicit set
-            const int n0 = n_act;
-            if (imp != -1) {
-                const int r = imp >= 0 ? imp : 256;
-                const int rb = g.wbi_off[r], rn = g.wbi_off[r + 1] - rb;
-                for (int t0 = 0; t0 < rn; t0 += 64) {
-                    const int t = t0 + lane_id();
-                    const bool ok = t < rn;
-                    const uint32_t v = ok ? g.wbi_dst[rb + t] : 0u;
-                    const bool nw = ok && set_new(cur, v);
-                    n_act = list_append<CAP>(cl, n_act, nw, v);
-                }
-            }
-            for (int k0 = 0; k0 < n0; k0 += 64) {
-                const int k = k0 + lane_id();
-                const uint32_t st = k < n0 ? (cl[k] & kIdMask) : 0u;
-                const uint32_t b = k < n0 ? g.wbs_off[st] : 0u;
-                const uint32_t e = k < n0 ? g.wbs_off[st + 1] : 0u;
-                for (uint32_t j = 0; __ballot(b + j < e); ++j) {
-                    const bool ok = b + j < e;
-                    const uint32_t v = ok ? g.wbs_dst[b + j] : 0u;
-                    const bool nw = ok && set_new(cur, v);
-                    n_act = list_append<CAP>(cl, n_act, nw, v);
-                }
-            }
-            n_act = uniform(n_act);
-            if (n_act > CAP) sparse = false;
-            wave_sync();
-            return;
-        }
-        materialize();
-        if (g.wb_parallel) {
-            // no destination is a source: one parallel pass, the wave's lanes
-            // over the edges
-            for (int t0 = 0; t0 < g.wb_count; t0 += 64) {
-                const int t = t0 + lane_id();
-                const bool ok = t < g.wb_count;
-                const uint2 e = ok ? g.list[g.wb_begin + t] : make_uint2(0u, 0u);
-                const bool nw = ok && test_bit(cur, e.x) && set_new(cur, e.y);
-                n_act = list_append<CAP>(cl, n_act, nw, e.y);
-            }
-            n_act = uniform(n_act);
-            if (n_act > CAP) sparse = false;
-            wave_sync();
-            return;
-        }
-        // in place, list order, one lane (thread_count == 1 semantics of
-        // src/iNFA_kernel.cu:145-164)
-        int n = n_act;
-        if (lane_id() == 0) {
-            for (int t = 0; t < g.wb_count; ++t) {
-                const uint2 e = g.list[g.wb_begin + t];
-                if (test_bit(cur, e.x)) {
-                    const uint64_t m = 1ull << (e.y & 63);
-                    if (!(cur[e.y >> 6] & m)) {
-                        cur[e.y >> 6] |= m;
-                        if (sparse && n < CAP) cl[n] = e.y;
-                        ++n;
-                    }
-                }
-            }
-        }
-        n_act = uniform(__builtin_amdgcn_readlane(n, 0));
-        if (n_act > CAP) sparse = false;
-        wave_sync();
-    };
-    int c_next = L > 0 ? win.at(0) : 0;
-    // a boundary before the first byte: the stream's previous byte (or the
-    // start, a non-word position) against it
-    if (has_wb && L > 0 && word_char(uint32_t(c_next)) != ((sflags & 2u) != 0)) wb_pass();
-
-    // two-byte rows of (byte, next byte), first 64 entries loaded a step ahead
-    auto load_pair = [&](int c1, int c2) -> RowSlice {
-        RowSlice r;
-        // read-only offsets: scalar loads
-        using CU32 = const __attribute__((address_space(4))) uint32_t;
-        const CU32* o = (CU32*)(g.pair_off) + (c1 * 256 + c2);
-        r.b = int(o[0]);
-        r.n = int(o[1]) - r.b;
-        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
-        return r;
-    };
-    RowSlice pair{0u, 0, 0};
-    const bool can_as = g.as_tab != nullptr;
-
-    StepCtx ctx;
-    ctx.len = L;
-    ctx.last_m = M_INFA;
-    ctx.c_prev = -1;
-    int cached = -1;
-    for (int b = 0; b < L; ++b) {
-        // the next byte is read one step ahead (two-byte rows, word boundaries)
-        const int c = c_next;
-        RowSlice pair_next{0u, 0, 0};
-        if (b + 1 < L) {
-            c_next = win.at(b + 1);
-            if (have_pair) pair_next = load_pair(c, c_next);
-        }
-        // active states including the implicit ones (selector features)
-        auto total_active = [&]() {
-            int n_tot = n_act;
-            if (imp >= 0) {
-                int ab, an;
-                alw_range(imp, ab, an);
-                n_tot += an;
-            } else if (imp == -2) {
-                n_tot += g.n_alw_states;
-            }
-            return n_tot;
-        };
-        ctx.c = c;
-        ctx.sparse = sparse;
-        ctx.pos = b;
-        bool log_step = false;
-        int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
-        // without two-byte rows the first positions of the implicit set are
-        // not reachable from the table: list them before an AS step
-        if (m == M_AS && !have_pair && imp >= 0) materialize();
-        if (m == M_AS && !(sparse && can_as)) m = M_INFA;
-        bool fsp;
-        int n_fut;
-        int next_imp = -1;
-        if (!log_step) {
-            if (m == M_AS) {
-                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl);
-                fsp = n_fut <= CAP;
-                next_imp = c;
-            } else {
-                if (MODE == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
-                materialize();
-                n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp);
-            }
-        } else {
-            // time every method on the same state: AS on the implicit form,
-            // then TKO and iNFA on the materialised set; keep the iNFA result
-            float tm[3] = {-1.f, -1.f, -1.f};
-            if (sparse && can_as) {
-                __builtin_amdgcn_s_waitcnt(0);
-                const uint64_t t0 = clock64();
-                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl);
-                fsp = n_fut <= CAP;
-                wave_sync();
-                __builtin_amdgcn_s_waitcnt(0);
-                tm[M_AS] = float(clock64() - t0);
-                clear_bits<CAP>(fut, fl, n_fut, fsp, W);
-                wave_sync();
-            }
-            materialize();
-            for (int mm = M_TKO; mm >= M_INFA; --mm) {
-                __builtin_amdgcn_s_waitcnt(0);
-                const uint64_t t0 = clock64();
-                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp);
-                wave_sync();
-                if (!fsp) {  // a bitmap-only result pays for its list rebuild
-                    n_fut = uniform(rebuild_list<CAP>(fut, W, fl));
-                    fsp = n_fut <= CAP;
-                }
-                wave_sync();
-                __builtin_amdgcn_s_waitcnt(0);
-                tm[mm] = float(clock64() - t0);
-                if (mm == M_TKO) {
-                    clear_bits<CAP>(fut, fl, n_fut, fsp, W);
-                    wave_sync();
-                }
-            }
-            const float fl = feature(g, ctx, lane_id());
-            unsigned long long slot = 0;
-            if (lane_id() == 0) slot = atomicAdd(a.log_top, 1ull);
-            slot = __shfl(static_cast<long long>(slot), 0, 64);
-            if (int64_t(slot) < a.log_cap) {
-                float* rec = a.log_buf + int64_t(slot) * kLogRecord;
-                const int l = lane_id();
-                if (l < kFeatures) rec[l] = fl;
-                if (l < 3) rec[kFeatures + l] = tm[l];
-                if (l == 0) rec[kFeatures + 3] = float(m);
-                if (l == 1) rec[kFeatures + 4] = float(c);
-                if (l == 2) rec[kFeatures + 5] = float(n_act);
-            }
-            m = M_INFA;
-        }
-        wave_sync();
-        clear_bits<CAP>(cur, cl, n_act, sparse, W);
-        // swap
-        uint64_t* tb = cur;
-        cur = fut;
-        fut = tb;
-        uint32_t* tl = cl;
-        cl = fl;
-        fl = tl;
-        wave_sync();
-        imp = next_imp;
-        ctx.c_prev = c;
-        if (fsp) {
-            n_act = n_fut;
-            sparse = true;
-        } else {
-            // bitmap only: fold the implicit set in, then count (and list
-            // the states if they fit)
-            n_act = 0;
-            sparse = false;
-            materialize();
-            n_act = uniform(rebuild_list<CAP>(cur, W, cl));
-            sparse = n_act <= CAP;
-        }
-        // constant indices keep the counters in registers (AS steps: the
-        // string length minus these)
-        msteps[0] += m == M_INFA ? 1 : 0;
-        msteps[1] += m == M_TKO ? 1 : 0;
-        ctx.last_m = m;
-        if (has_wb) {
-            const uint32_t xb = uint32_t(c);
-            // a stream defers its last byte's boundary to the next chunk
-            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(uint32_t(c_next)))
-                                              : (slot < 0 && word_char(xb));
-            if (boundary) wb_pass();
-        }
-        pair = pair_next;
-    }
-    materialize();
-    wave_sync();
-    if (slot >= 0) {
-        // save the set before the end-of-string boundary, then report as if
-        // the stream ended here
-        uint64_t* out = a.stream_out_state + size_t(sid) * W;
-        for (int j = lane_id(); j < W; j += 64) out[j] = cur[j];
-        const bool last_word = L > 0 ? word_char(uint32_t(c_next)) : ((sflags & 2u) != 0);
-        if (lane_id() == 0) a.stream_out_flags[sid] = uint8_t(1u | (last_word ? 2u : 0u));
-        if (has_wb && last_word) wb_pass();
-        wave_sync();
-    }
-    emit_accepts(g, a, sid, cur);
-    wave_sync();
-    clear_bits<CAP>(cur, cl, n_act, sparse, W);
-    wave_sync();
-}
-
-// ------------------------------------------------------------ hashed storage
-// The same step loop with the set held as the active list plus an LDS hash
-// table of 2*CAP slots per list (instead of two S-bit bitmaps), so a wave's
-// LDS no longer grows with the rule set and many more waves fit on a CU.
-// Two S-bit bitmaps per wave live in HBM (ScanArgs::dense) and are touched
-// only by dense steps: a set of more than CAP states, a step whose method is
-// iNFA or TKO, an AS step that overflowed the list (redone as iNFA), and
-// acceptance.  Invariants between steps: the fut table is empty and the fut
-// bitmap is zero; a sparse set has its states in cl and hc (cur bitmap zero),
-// a dense set in the cur bitmap (hc empty).
-
-// Orders this wave's global-memory accesses across lanes (the HBM bitmaps).
-__device__ __forceinline__ void dense_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-}
-
-__device__ int count_bits(const uint64_t* bits, int W) {
-    int n = 0;
-    for (int j = lane_id(); j < W; j += 64) n += __popcll(bits[j]);
-    return uniform(wave_sum(n));
-}
-
-// iNFA or TKO step on whole bitmaps (fut zero on entry).
-__device__ void dense_method(const DevGraph& g, int m, int c, const uint64_t* cur, uint64_t* fut) {
-    if (m == M_TKO) {
-        tko_groups(g, c, cur, fut);
-        dense_sync();
-        scatter<64, false>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, nullptr, 0);
-    } else {
-        for (int j = lane_id(); j < g.W; j += 64) fut[j] = cur[j] & g.persis[j];
-        dense_sync();
-        scatter<64, false>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, nullptr, 0);
-    }
-    dense_sync();
-}
-
-template <int CAP, int MODE>
-__device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* gA, uint64_t* gB,
-                              uint32_t* hA, uint32_t* hB, uint32_t* listA, uint32_t* listB, int* msteps) {
-    using HS = HashSet<2 * CAP>;
-    // the selector mode, and whether the boundary code and the path for
-    // missing two-byte rows are compiled in
-    constexpr int M = MODE & ~kModeLean;
-    constexpr bool kLean = (MODE & kModeLean) != 0;
-    const int W = g.W;
-    const int64_t beg = a.offsets[sid];
-    const int L = int(a.offsets[sid + 1] - beg);
-    msteps[3] += L;
-    uint64_t* gcur = gA;
-    uint64_t* gfut = gB;
-    HS hc{hA}, hf{hB};
-    uint32_t* cl = listA;
-    uint32_t* fl = listB;
-    // the always-active states start implicit, the other initial states listed
-    int n_act = uniform(rebuild_list<CAP>(g.init_explicit, W, cl));
-    bool sparse = n_act <= CAP;
-    if (sparse) {
-        for (int k = lane_id(); k < n_act; k += 64) hc.insert(cl[k]);
-    } else {
-        for (int j = lane_id(); j < W; j += 64) gcur[j] = g.init_explicit[j];
-        dense_sync();
-    }
-    wave_sync();
-    ByteWindow win{a.text + beg, L, 0, 0u};
-    if (L > 0) win.load(0);
-    const bool has_wb = !kLean && g.wb_count > 0;
-    const bool have_pair = kLean || g.pair_off != nullptr;
-    int imp = -2;  // implicitly held states, as in scan_string
-    auto alw_range = [&](int cc, int& b0, int& n0) {
-        b0 = g.sym_stats[cc].w;
-        n0 = (cc < 255 ? g.sym_stats[cc + 1].w : g.alw_total) - b0;
-    };
-    // sparse -> dense: the listed states into the cur bitmap
-    auto to_dense = [&]() {
-        if (!sparse) return;
-        for (int k = lane_id(); k < n_act; k += 64) set_bit_atomic(gcur, cl[k] & kIdMask);
-        hc.clear();
-        sparse = false;
-        dense_sync();
-        wave_sync();
-    };
-    auto materialize = [&]() {
-        if (imp == -1) return;
-        int ab = 0, an = g.n_alw_states;
-        const uint32_t* src = g.alw_states;
-        if (imp >= 0) {
-            alw_range(imp, ab, an);
-            src = g.alw_dst;
-        }
-        if (sparse && n_act + an <= CAP) {
-            for (int t0 = 0; t0 < an; t0 += 64) {
-                const int t = t0 + lane_id();
-                const bool ok = t < an;
-                const uint32_t v = ok ? (imp >= 0 ? src[ab + t] : (src[t] | kTag)) : 0u;
-                const bool nw = ok && hc.insert(v & kIdMask);
-                n_act = list_append<CAP>(cl, n_act, nw, v & (kIdMask | kTag));
-            }
-            n_act = uniform(n_act);
-            wave_sync();
-        } else {
-            to_dense();
-            for (int t = lane_id(); t < an; t += 64) set_bit_atomic(gcur, src[ab + t] & kIdMask);
-            dense_sync();
-            n_act = count_bits(gcur, W);
-        }
-        imp = -1;
-    };
-    auto wb_pass = [&]() {
-        materialize();
-        if (g.wb_parallel) {
-            // no destination is a source: one parallel pass.  The edges whose
-            // source is active bound the states it adds; the list form is kept
-            // when they fit.
-            int hits = CAP + 1;
-            if (sparse) {
-                hits = 0;
-                for (int t0 = 0; t0 < g.wb_count; t0 += 64) {
-                    const int t = t0 + lane_id();
-                    const bool ok = t < g.wb_count;
-                    const uint2 e = ok ? g.list[g.wb_begin + t] : make_uint2(0u, 0u);
-                    hits += __popcll(__ballot(ok && hc.contains(e.x)));
-                }
-            }
-            if (n_act + hits <= CAP) {
-                for (int t0 = 0; t0 < g.wb_count; t0 += 64) {
-                    const int t = t0 + lane_id();
-                    const bool ok = t < g.wb_count;
-                    const uint2 e = ok ? g.list[g.wb_begin + t] : make_uint2(0u, 0u);
-                    const bool nw = ok && hc.contains(e.x) && hc.insert(e.y);
-                    n_act = list_append<CAP>(cl, n_act, nw, e.y);
-                }
-                n_act = uniform(n_act);
-                wave_sync();
-            } else {
-                to_dense();
-                for (int t0 = 0; t0 < g.wb_count; t0 += 64) {
-                    const int t = t0 + lane_id();
-                    if (t < g.wb_count) {
-                        const uint2 e = g.list[g.wb_begin + t];
-                        if (test_bit(gcur, e.x)) set_bit_atomic(gcur, e.y);
-                    }
-                }
-                dense_sync();
-                n_act = count_bits(gcur, W);
-            }
-            return;
-        }
-        // in place, list order, one lane (thread_count == 1 semantics of
-        // src/iNFA_kernel.cu:145-164)
-        if (sparse && n_act + g.wb_count <= CAP) {
-            int n = n_act;
-            if (lane_id() == 0) {
-                for (int t = 0; t < g.wb_count; ++t) {
-                    const uint2 e = g.list[g.wb_begin + t];
-                    if (hc.contains(e.x) && hc.insert(e.y)) cl[n++] = e.y;
-                }
-            }
-            n_act = uniform(__builtin_amdgcn_readlane(n, 0));
-            wave_sync();
-        } else {
-            to_dense();
-            if (lane_id() == 0) {
-                for (int t = 0; t < g.wb_count; ++t) {
-                    const uint2 e = g.list[g.wb_begin + t];
-                    if (test_bit(gcur, e.x)) gcur[e.y >> 6] |= 1ull << (e.y & 63);
-                }
-            }
-            dense_sync();
-            n_act = count_bits(gcur, W);
-        }
-    };
-    int c_next = L > 0 ? win.at(0) : 0;
-    if (has_wb && L > 0 && word_char(uint32_t(c_next))) wb_pass();
-
-    auto load_pair = [&](int c1, int c2) -> RowSlice {
-        RowSlice r;
-        // read-only offsets: scalar loads
-        using CU32 = const __attribute__((address_space(4))) uint32_t;
-        // a 32-bit byte offset lets the load take it as its scalar offset
-        const uint32_t off = ((uint32_t(c1) << 8) | uint32_t(c2)) << 2;
-        const CU32* o = (const CU32*)((const __attribute__((address_space(4))) char*)(g.pair_off) + off);
-        r.b = int(o[0]);
-        r.n = int(o[1]) - r.b;
-        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
-        return r;
-    };
-    RowSlice pair{0u, 0, 0};
-    const bool can_as = g.as_tab != nullptr;
-
-    StepCtx ctx;
-    ctx.len = L;
-    ctx.last_m = M_INFA;
-    ctx.c_prev = -1;
-    int cached = -1;
-    for (int b = 0; b < L; ++b) {
-        // the next byte is read one step ahead (two-byte rows, word boundaries)
-        // (at the last byte the window reads 0 past the end and the row is unused)
-        const int c = c_next;
-        RowSlice pair_next{0u, 0, 0};
-        c_next = win.at(b + 1);
-        if (have_pair) pair_next = load_pair(c, c_next);
-        auto total_active = [&]() {
-            int n_tot = n_act;
-            if (imp >= 0) {
-                int ab, an;
-                alw_range(imp, ab, an);
-                n_tot += an;
-            } else if (imp == -2) {
-                n_tot += g.n_alw_states;
-            }
-            return n_tot;
-        };
-        ctx.c = c;
-        ctx.sparse = sparse;
-        ctx.pos = b;
-        bool log_step = false;
-        int m = choose_method<M>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
-        if (m == M_AS && !(sparse && can_as)) m = M_INFA;
-        int n_fut = 0;
-        bool fsp = false;
-        if (m == M_AS) {
-            n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, hf, fl);
-            wave_sync();
-            if (n_fut <= CAP) {
-                fsp = true;
-            } else {
-                // the list overflowed: drop the partial result, redo as iNFA
-                hf.clear();
-                wave_sync();
-                m = M_INFA;
-            }
-        }
-        if (fsp) {
-            hc.clear();  // retire cur (its bitmap is zero)
-            HS th = hc;
-            hc = hf;
-            hf = th;
-            uint32_t* tl = cl;
-            cl = fl;
-            fl = tl;
-            n_act = n_fut;
-            imp = c;
-            wave_sync();
-            // without two-byte rows the next step could not reach the first
-            // positions of the implicit set from the table: list them now
-            if (!have_pair) materialize();
-        } else {
-            if (M == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
-            // bitmap steps counted here, off the AS path (AS steps: the string
-            // length minus these)
-            msteps[0] += m == M_TKO ? 0 : 1;
-            msteps[1] += m == M_TKO ? 1 : 0;
-            materialize();
-            to_dense();
-            dense_method(g, m, c, gcur, gfut);
-            for (int j = lane_id(); j < W; j += 64) gcur[j] = 0ull;  // retire cur
-            uint64_t* tb = gcur;
-            gcur = gfut;
-            gfut = tb;
-            dense_sync();
-            n_act = uniform(rebuild_list<CAP>(gcur, W, cl));
-            if (n_act <= CAP) {
-                // back to the list form
-                for (int k = lane_id(); k < n_act; k += 64) {
-                    hc.insert(cl[k]);
-                    gcur[cl[k] >> 6] = 0ull;
-                }
-                sparse = true;
-                dense_sync();
-            }
-            wave_sync();
-            imp = -1;
-        }
-        ctx.c_prev = c;
-        ctx.last_m = m;
-        if (has_wb) {
-            const uint32_t xb = uint32_t(c);
-            const bool boundary = (b < L - 1) ? (word_char(xb) != word_char(uint32_t(c_next))) : word_char(xb);
-            if (boundary) wb_pass();
-        }
-        pair = pair_next;
-    }
-    // accepted states of a valid list: the listed ones and the implicitly
-    // held ones (not in the table), compacted into the free list; up to 64
-    // are emitted by rank without touching the bitmaps
-    int cnt = -1;
-    if (sparse) {
-        cnt = 0;
-        for (int k0 = 0; k0 < n_act; k0 += 64) {
-            const int k = k0 + lane_id();
-            const uint32_t id = k < n_act ? (cl[k] & kIdMask) : 0u;
-            cnt = list_append<CAP>(fl, cnt, k < n_act && test_bit(g.accept, id), id);
-        }
-        if (imp != -1) {
-            int ab = 0, an = g.n_alw_states;
-            const uint32_t* src = g.alw_states;
-            if (imp >= 0) {
-                alw_range(imp, ab, an);
-                src = g.alw_dst;
-            }
-            for (int t0 = 0; t0 < an; t0 += 64) {
-                const int t = t0 + lane_id();
-                const uint32_t id = t < an ? (src[ab + t] & kIdMask) : 0u;
-                cnt = list_append<CAP>(fl, cnt, t < an && test_bit(g.accept, id) && !hc.contains(id), id);
-            }
-        }
-        cnt = uniform(cnt);
-        wave_sync();
-    }
-    if (cnt == 0) {
-        if (lane_id() == 0) {
-            a.res_count[sid] = 0;
-            a.res_start[sid] = 0;
-        }
-        hc.clear();
-    } else if (cnt > 0 && cnt <= 64) {
-        // ascending by rank: lane l's id is larger than `rank` others (the
-        // ids are distinct)
-        const uint32_t v = lane_id() < cnt ? fl[lane_id()] : 0xffffffffu;
-        int rank = 0;
-        for (int j = 0; j < cnt; ++j) rank += uint32_t(__builtin_amdgcn_readlane(int(v), j)) < v ? 1 : 0;
-        unsigned long long s0 = 0;
-        if (lane_id() == 0) s0 = atomicAdd(a.pool_top, (unsigned long long)cnt);
-        const int64_t start = int64_t(__shfl(static_cast<long long>(s0), 0, 64));
-        int total = cnt;
-        if (start + cnt > a.pool_cap) {
-            if (lane_id() == 0) atomicOr(a.overflow, 1);
-            total = -cnt;  // marks a truncated result
-        } else if (lane_id() < cnt) {
-            a.res_pool[start + rank] = v;
-        }
-        if (lane_id() == 0) {
-            a.res_count[sid] = total;
-            a.res_start[sid] = start;
-        }
-        hc.clear();
-    } else {
-        // many accepted states, or a bitmap-only set: the whole set, then the
-        // accepted states through a bitmap in ascending order
-        materialize();
-        if (sparse) {
-            for (int k = lane_id(); k < n_act; k += 64) {
-                const uint32_t id = cl[k] & kIdMask;
-                if (test_bit(g.accept, id)) set_bit_atomic(gfut, id);
-            }
-            dense_sync();
-            emit_accepts(g, a, sid, gfut);
-            dense_sync();
-            for (int k = lane_id(); k < n_act; k += 64) gfut[(cl[k] & kIdMask) >> 6] = 0ull;
-            dense_sync();
-            hc.clear();
-        } else {
-            emit_accepts(g, a, sid, gcur);
-            dense_sync();
-            for (int j = lane_id(); j < W; j += 64) gcur[j] = 0ull;
-            dense_sync();
-        }
-    }
-    wave_sync();
-}
-
-template <int CAP>
-struct LaneH {
-    // u64 words per wave: two lists of CAP + 64 u32 and two tables of
-    // 2*CAP + 64 u32 (each followed by its 64 sink entries)
-    static constexpr size_t words() { return size_t(3) * CAP + 128; }
-};
-
-// Build knob: ADM_HASHED_WAVES > 0 asks the compiler to fit that many waves
-// per SIMD in registers (it may spill to scratch to do so).
-#ifndef ADM_HASHED_WAVES
-#define ADM_HASHED_WAVES 0
-#endif
-#if ADM_HASHED_WAVES > 0
-#define ADM_HASHED_ATTR __attribute__((amdgpu_waves_per_eu(ADM_HASHED_WAVES)))
-#else
-#define ADM_HASHED_ATTR
-#endif
-
-// The per-byte statistics stay in device memory here (they are read off the
-// AS fast path only), keeping the workgroup's LDS to the waves' own.
-template <int WPB, int CAP, int MODE>
-__global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevGraph g, ScanArgs a) {
-    extern __shared__ uint64_t lds[];
-    const int wave = threadIdx.x >> 6;
-    uint64_t* base = lds + LaneH<CAP>::words() * wave;
-    uint32_t* listA = reinterpret_cast<uint32_t*>(base);
-    uint32_t* listB = listA + CAP + 64;
-    uint32_t* hA = listB + CAP + 64;
-    uint32_t* hB = hA + HashSet<2 * CAP>::kSlots;
-    uint64_t* gA = a.dense + (size_t(blockIdx.x) * WPB + wave) * 2 * size_t(g.W);
-    uint64_t* gB = gA + g.W;
-    HashSet<2 * CAP>{hA}.clear();
-    HashSet<2 * CAP>{hB}.clear();
-    HashSet<2 * CAP>{hA}.init_sink();
-    HashSet<2 * CAP>{hB}.init_sink();
-    wave_sync();
-    int msteps[4] = {0, 0, 0, 0};  // iNFA, TKO, AS (filled below), bytes scanned
-    for (;;) {
-        int idx = 0;
-        if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
-        idx = uniform(__builtin_amdgcn_readlane(idx, 0));
-        if (idx >= a.n) break;
-        const int sid = a.order ? a.order[idx] : idx;
-        scan_string_h<CAP, MODE>(g, a, sid, gA, gB, hA, hB, listA, listB, msteps);
-    }
-    msteps[2] = msteps[3] - msteps[0] - msteps[1];
-    if (a.method_steps != nullptr && lane_id() == 0)
-        for (int m = 0; m < 3; ++m)
-            if (msteps[m]) atomicAdd(&a.method_steps[m], (unsigned long long)msteps[m]);
-}
-
-template <int WPB, int CAP, int MODE>
-__global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a) {
-    extern __shared__ uint64_t lds[];
-    // per-byte statistics shared by the workgroup's waves
-    __shared__ int4 s_stats[256];
-    for (int i = threadIdx.x; i < 256; i += 64 * WPB) s_stats[i] = g.sym_stats[i];
-    __syncthreads();
-    g.sym_stats = s_stats;
-    const int wave = threadIdx.x >> 6;
-    const size_t per = Lane<CAP>::words(g.W);
-    uint64_t* base = lds + per * wave;
-    uint64_t* bitsA = base;
-    uint64_t* bitsB = base + g.W;
-    uint32_t* listA = reinterpret_cast<uint32_t*>(base + 2 * g.W);
-    uint32_t* listB = listA + CAP;
-    for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
-    wave_sync();
-    int msteps[4] = {0, 0, 0, 0};  // iNFA, TKO, AS (filled below), bytes scanned
-    for (;;) {
-        int idx = 0;
-        if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
-        idx = uniform(__builtin_amdgcn_readlane(idx, 0));
-        if (idx >= a.n) break;
-        const int sid = a.order ? a.order[idx] : idx;
-        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, msteps);
-    }
-    msteps[2] = msteps[3] - msteps[0] - msteps[1];
-    if (a.method_steps != nullptr && lane_id() == 0)
-        for (int m = 0; m < 3; ++m)
-            if (msteps[m]) atomicAdd(&a.method_steps[m], (unsigned long long)msteps[m]);
 }
 
 // ------------------------------------------------------------ ordering
@@ -1520,75 +133,9 @@ __global__ void bin_scatter_kernel(const int64_t* offsets, int32_t n, const int3
     }
 }
 
+
 // ------------------------------------------------------------ launchers
 namespace {
-// dynamic LDS the engine may request: the CU's 160 KB minus the static
-// per-workgroup byte statistics
-constexpr int kEngineDynLds = 160 * 1024 - int(sizeof(int4)) * 256;
-
-template <int WPB, int CAP, int MODE, bool HASHED>
-const void* engine_fn() {
-    if constexpr (HASHED) return reinterpret_cast<const void*>(&engine_h_kernel<WPB, CAP, MODE>);
-    else return reinterpret_cast<const void*>(&engine_kernel<WPB, CAP, MODE>);
-}
-template <int WPB, int CAP, int MODE, bool HASHED>
-hipError_t engine_launch_m(int grid, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute(engine_fn<WPB, CAP, MODE, HASHED>(), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  kEngineDynLds);
-        (void)hipGetLastError();  // an unsupported attribute must not fail the launch check
-        attr = true;
-    }
-    if constexpr (HASHED) {
-        hipLaunchKernelGGL((engine_h_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
-    } else {
-        hipLaunchKernelGGL((engine_kernel<WPB, CAP, MODE>), dim3(grid), dim3(64 * WPB), lds, st, g, a);
-    }
-    return hipGetLastError();
-}
-template <int WPB, int CAP, int MODE, bool HASHED>
-hipError_t engine_occ_m(size_t lds, int* blocks) {
-    (void)hipFuncSetAttribute(engine_fn<WPB, CAP, MODE, HASHED>(), hipFuncAttributeMaxDynamicSharedMemorySize,
-                              kEngineDynLds);
-    (void)hipGetLastError();
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, engine_fn<WPB, CAP, MODE, HASHED>(), 64 * WPB, lds);
-}
-// hashed storage is built for the modes that run mostly AS steps
-#define ADM_DISPATCH_MODE(FN, ...)                                                      \
-    if (hashed) {                                                                       \
-        switch (mode) {                                                                 \
-            case MODE_AS: return FN<WPB, CAP, MODE_AS, true>(__VA_ARGS__);              \
-            case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, true>(__VA_ARGS__);    \
-            case MODE_LEARNED_AS: return FN<WPB, CAP, MODE_LEARNED_AS, true>(__VA_ARGS__); \
-            case MODE_AS | kModeLean: return FN<WPB, CAP, MODE_AS | kModeLean, true>(__VA_ARGS__); \
-            case MODE_LEARNED | kModeLean: return FN<WPB, CAP, MODE_LEARNED | kModeLean, true>(__VA_ARGS__); \
-            case MODE_LEARNED_AS | kModeLean:                                           \
-                return FN<WPB, CAP, MODE_LEARNED_AS | kModeLean, true>(__VA_ARGS__);    \
-            default: return hipErrorInvalidValue;                                       \
-        }                                                                               \
-    }                                                                                   \
-    switch (mode) {                                                                     \
-        case MODE_INFA: return FN<WPB, CAP, MODE_INFA, false>(__VA_ARGS__);             \
-        case MODE_TKO: return FN<WPB, CAP, MODE_TKO, false>(__VA_ARGS__);               \
-        case MODE_AS: return FN<WPB, CAP, MODE_AS, false>(__VA_ARGS__);                 \
-        case MODE_ADMATCH: return FN<WPB, CAP, MODE_ADMATCH, false>(__VA_ARGS__);       \
-        case MODE_HEUR: return FN<WPB, CAP, MODE_HEUR, false>(__VA_ARGS__);             \
-        case MODE_LEARNED: return FN<WPB, CAP, MODE_LEARNED, false>(__VA_ARGS__);       \
-        case MODE_LOG: return FN<WPB, CAP, MODE_LOG, false>(__VA_ARGS__);               \
-        case MODE_LEARNED_EVERY: return FN<WPB, CAP, MODE_LEARNED_EVERY, false>(__VA_ARGS__); \
-        case MODE_LEARNED_AS: return FN<WPB, CAP, MODE_LEARNED_AS, false>(__VA_ARGS__); \
-        default: return hipErrorInvalidValue;                                           \
-    }
-template <int WPB, int CAP>
-hipError_t engine_launch_t(int mode, bool hashed, int grid, size_t lds, hipStream_t st, const DevGraph& g,
-                           const ScanArgs& a) {
-    ADM_DISPATCH_MODE(engine_launch_m, grid, lds, st, g, a)
-}
-template <int WPB, int CAP>
-hipError_t engine_occ_t(int mode, bool hashed, size_t lds, int* blocks) {
-    ADM_DISPATCH_MODE(engine_occ_m, lds, blocks)
-}
 template <int BS>
 hipError_t plain_launch_t(int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
     static bool attr = false;
@@ -1603,30 +150,26 @@ hipError_t plain_launch_t(int n, size_t lds, hipStream_t st, const DevGraph& g, 
 }
 }  // namespace
 
-#define ADM_DISPATCH_ENGINE(FN, ...)                                   \
-    switch (wpb * 10000 + cap) {                                       \
-        case 10128: return FN<1, 128>(__VA_ARGS__);                    \
-        case 20128: return FN<2, 128>(__VA_ARGS__);                    \
-        case 40128: return FN<4, 128>(__VA_ARGS__);                    \
-        case 10256: return FN<1, 256>(__VA_ARGS__);                    \
-        case 20256: return FN<2, 256>(__VA_ARGS__);                    \
-        case 40256: return FN<4, 256>(__VA_ARGS__);                    \
-        case 10512: return FN<1, 512>(__VA_ARGS__);                    \
-        case 20512: return FN<2, 512>(__VA_ARGS__);                    \
-        case 40512: return FN<4, 512>(__VA_ARGS__);                    \
-        case 11024: return FN<1, 1024>(__VA_ARGS__);                   \
-        case 21024: return FN<2, 1024>(__VA_ARGS__);                   \
-        case 41024: return FN<4, 1024>(__VA_ARGS__);                   \
-        default: return hipErrorInvalidValue;                          \
+#define ADM_DISPATCH_CAP(FN, ...)                                  \
+    switch (cap * 2 + int(hashed)) {                               \
+        case 256: return FN<128, false>(__VA_ARGS__);              \
+        case 257: return FN<128, true>(__VA_ARGS__);               \
+        case 512: return FN<256, false>(__VA_ARGS__);              \
+        case 513: return FN<256, true>(__VA_ARGS__);               \
+        case 1024: return FN<512, false>(__VA_ARGS__);             \
+        case 1025: return FN<512, true>(__VA_ARGS__);              \
+        case 2048: return FN<1024, false>(__VA_ARGS__);            \
+        case 2049: return FN<1024, true>(__VA_ARGS__);             \
+        default: return hipErrorInvalidValue;                      \
     }
 
 hipError_t launch_engine(int wpb, int cap, int mode, bool hashed, int grid, size_t lds, hipStream_t st,
                          const DevGraph& g, const ScanArgs& a) {
-    ADM_DISPATCH_ENGINE(engine_launch_t, mode, hashed, grid, lds, st, g, a)
+    ADM_DISPATCH_CAP(engine_launch_cap, wpb, mode, grid, lds, st, g, a)
 }
 
 hipError_t engine_occupancy(int wpb, int cap, int mode, bool hashed, size_t lds, int* blocks) {
-    ADM_DISPATCH_ENGINE(engine_occ_t, mode, hashed, lds, blocks)
+    ADM_DISPATCH_CAP(engine_occ_cap, wpb, mode, lds, blocks)
 }
 
 hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a) {
@@ -1634,10 +177,11 @@ hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGrap
         case 64: return plain_launch_t<64>(n, lds, st, g, a);
         case 128: return plain_launch_t<128>(n, lds, st, g, a);
         case 256: return plain_launch_t<256>(n, lds, st, g, a);
+        case 512: return plain_launch_t<512>(n, lds, st, g, a);
+        case 1024: return plain_launch_t<1024>(n, lds, st, g, a);
         default: return hipErrorInvalidValue;
     }
 }
-
 hipError_t launch_policy(const DevGraph& g, const float* mlp, uint32_t* table, hipStream_t st) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<int32_t*>(table), 0,
                        int64_t(kPolicyWords));

@@ -1,4 +1,4 @@
-// Host entry points of the kernels in kernels.hip.
+// Host entry points of the kernels in kernels.hip and engine_c*_*.hip.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,9 +9,21 @@ namespace admatch {
 
 constexpr int kLenBins = 40;
 
+// dynamic LDS the engine may request: the CU's 160 KB minus the static
+// per-workgroup byte statistics
+constexpr int kEngineDynLds = 160 * 1024 - int(sizeof(int4)) * 256;
+
 hipError_t launch_engine(int wpb, int cap, int mode, bool hashed, int grid, size_t lds, hipStream_t st,
                          const DevGraph& g, const ScanArgs& a);
 hipError_t engine_occupancy(int wpb, int cap, int mode, bool hashed, size_t lds, int* blocks);
+// per capacity and storage (defined in engine_c<CAP>_<storage>.hip)
+template <int CAP, bool HASHED>
+hipError_t engine_launch_cap(int wpb, int mode, int grid, size_t lds, hipStream_t st, const DevGraph& g,
+                             const ScanArgs& a);
+template <int CAP, bool HASHED>
+hipError_t engine_occ_cap(int wpb, int mode, size_t lds, int* blocks);
+
+// plain single-kernel iNFA path, workgroup size bs in {64, 128, 256, 512, 1024}
 hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGraph& g, const ScanArgs& a);
 hipError_t launch_fill(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 // streaming: copies row i of out_state/out_flags to slot ids[i] of state/flags
