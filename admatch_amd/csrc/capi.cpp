@@ -68,19 +68,37 @@ int status(int rc, const std::string& err) {
     return rc;
 }
 
+// PROFILER_MODE / SHOW_RESULTS output in the reference's format
+// (src/kernel_common.cu:36-95).  Device buffers are resident and reused, so
+// "Device MemAlloc" is the time spent growing them (usually 0) and "Device
+// MemFree" is 0; "Accept Map" is the host's state-to-rule lookup.
 void maybe_report(ita_handle* h, int32_t n, double wall_ms) {
-    if (h->flags & ITA_PROFILER_MODE) {
-        const auto& st = h->eng.stats();
-        std::printf("%20s%10.4f ms\n%20s%10.6f ms\n%20s%10.4f ms\n%20s%10.6f ms\n", "Total: ", wall_ms,
-                    "Per-string: ", n ? wall_ms / n : 0.0, "Kernel Exec: ", double(st.kernel_ms),
-                    "Per-string Kernel: ", n ? double(st.kernel_ms) / n : 0.0);
-    }
     if (h->flags & ITA_SHOW_RESULTS) {
+        const auto& c = h->eng.default_context();
         for (int32_t i = 0; i < n; ++i) {
-            std::printf("==================\nString [%d] :\nAccept Rules : ", i + 1);
+            std::printf("==================\nString [%d] :\nFinal States : ", i + 1);
+            if (size_t(i) + 1 < c.fin_off.size())
+                for (int32_t k = c.fin_off[size_t(i)]; k < c.fin_off[size_t(i) + 1]; ++k)
+                    std::printf("%u ", c.fin_states[size_t(k)]);
+            std::printf("\nAccept Rules : ");
             for (int32_t k = h->off[i]; k < h->off[i + 1]; ++k) std::printf("%d ", h->ids[k]);
             std::printf("\n==================\n\n");
         }
+    }
+    if (h->flags & ITA_PROFILER_MODE) {
+        const auto& st = h->eng.stats();
+        const double dev = double(st.alloc_ms) + st.h2d_ms + st.kernel_ms + st.d2h_ms;
+        auto row = [](const char* what, double ms) { std::printf("%20s%10.4f ms\n", what, ms); };
+        row("Total: ", wall_ms);
+        row("Per-string: ", n ? wall_ms / n : 0.0);
+        row("Device MemAlloc: ", st.alloc_ms);
+        row("MemCpy H2D: ", st.h2d_ms);
+        row("Kernel Exec: ", st.kernel_ms);
+        row("Per-string Kernel: ", n ? double(st.kernel_ms) / n : 0.0);
+        row("MemCpy D2H: ", st.d2h_ms);
+        row("Device MemFree: ", 0.0);
+        row("Accept Map: ", st.map_ms);
+        row("Device time: ", dev);
     }
     std::fflush(stdout);
 }
@@ -131,7 +149,8 @@ int ita_batched_scan(ita_handle* h, const uint8_t* const* text, const int32_t* l
     if (n < 0 || (n > 0 && (!out_offsets || (cap > 0 && !out_ids)))) return fail(ITA_ERR_ARG, "bad output buffers");
     std::string err;
     const auto t0 = std::chrono::steady_clock::now();
-    int rc = h->eng.scan_host(text, len, n, admatch::KernelKind(h->kind), h->mode, h->off, h->ids, err);
+    int rc = h->eng.scan_host(text, len, n, admatch::KernelKind(h->kind), h->mode, h->off, h->ids, err,
+                              (h->flags & ITA_SHOW_RESULTS) != 0);
     if (rc) return fail(rc, err);
     const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (out_offsets) std::memcpy(out_offsets, h->off.data(), sizeof(int32_t) * (size_t(n) + 1));
@@ -147,9 +166,10 @@ int ita_scan(ita_handle* h, const char* text, int32_t* out_ids, int32_t cap, int
     if (!text) return fail(ITA_ERR_ARG, "null text");
     const uint8_t* p = reinterpret_cast<const uint8_t*>(text);
     const int32_t len = int32_t(std::strlen(text));  // Interface.cu:155
-    int32_t off[2];
+    int32_t off[2] = {0, 0};
     int rc = ita_batched_scan(h, &p, &len, 1, off, cap > 0 ? out_ids : nullptr, cap);
-    if (n_out) *n_out = off[1];
+    // the count is known on success and when only the output was too small
+    if (n_out) *n_out = (rc == ITA_OK || rc == ITA_ERR_CAPACITY) ? off[1] : 0;
     return rc;
 }
 

@@ -74,6 +74,7 @@ struct ScanArgs {
     uint64_t* stream_out_state;
     uint8_t* stream_out_flags;
     int32_t policy_sparse, policy_dense;  // the table's method for every entry of that list validity, or -1
+    int32_t emit_all;         // write every final state, not only the accepting ones (SHOW_RESULTS)
 };
 
 enum SelectMode : int32_t {

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <limits>
 #include <cstring>
 
@@ -37,6 +38,8 @@ Context::~Context() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (done) (void)hipEventDestroy(done);
+    for (hipEvent_t e : {ev_h2d0, ev_h2d1, ev_d2h0, ev_d2h1})
+        if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -47,6 +50,10 @@ int Context::init(std::string& err) {
     ADM_HIP(hipEventCreate(&ev0));
     ADM_HIP(hipEventCreate(&ev1));
     ADM_HIP(hipEventCreateWithFlags(&done, hipEventDisableTiming));
+    ADM_HIP(hipEventCreate(&ev_h2d0));
+    ADM_HIP(hipEventCreate(&ev_h2d1));
+    ADM_HIP(hipEventCreate(&ev_d2h0));
+    ADM_HIP(hipEventCreate(&ev_d2h1));
     ADM_HIP(hipMalloc(&d_ctr, kCtrWords * sizeof(int32_t)));
     ADM_HIP(hipHostMalloc(&h_ctr, kCtrWords * sizeof(int32_t)));
     return ADM_OK;
@@ -356,6 +363,7 @@ ScanArgs Engine::make_args(Context& c, const uint8_t* d_text, const int64_t* d_o
     a.log_top = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrLogTop);
     a.log_stride = log_stride_ > 0 ? log_stride_ : 1;
     a.method_steps = reinterpret_cast<unsigned long long*>(c.d_ctr + kCtrMethods);
+    a.emit_all = c.emit_all ? 1 : 0;
     return a;
 }
 
@@ -465,10 +473,12 @@ int Engine::scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off,
         return ADM_ERR_ARG;
     }
     if (int rc = c.init(err)) return rc;
+    const auto ta = std::chrono::steady_clock::now();
     if (int rc = c.ensure_results(n, err)) return rc;
     if (kind == KIND_ENGINE) mode = kernel_mode(mode);
     if (kind == KIND_ENGINE && n > 0 && !c.active_stream)
         if (int rc = prepare_dense(c, mode, err)) return rc;
+    c.stats.alloc_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - ta).count();
     hipStream_t st = c.stream;
     ADM_HIP(hipEventRecord(c.ev0, st));
     if (use_graph && n > 0) {
@@ -523,7 +533,9 @@ int Engine::stage(Context& c, const uint8_t* const* text, const int32_t* len, in
     if (int rc = c.init(err)) return rc;
     // the previous use of the staging buffers must have finished
     ADM_HIP(hipStreamSynchronize(c.stream));
+    const auto ta = std::chrono::steady_clock::now();
     if (int rc = c.ensure_staging(total, n, err)) return rc;
+    c.stats.alloc_ms += std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - ta).count();
     int64_t o = 0;
     for (int32_t i = 0; i < n; ++i) {
         c.h_off[i] = o;
@@ -531,8 +543,10 @@ int Engine::stage(Context& c, const uint8_t* const* text, const int32_t* len, in
         o += len[i];
     }
     c.h_off[n] = o;
+    ADM_HIP(hipEventRecord(c.ev_h2d0, c.stream));
     if (total > 0) ADM_HIP(hipMemcpyAsync(c.d_text, c.h_text, size_t(total), hipMemcpyHostToDevice, c.stream));
     ADM_HIP(hipMemcpyAsync(c.d_off, c.h_off, sizeof(int64_t) * (size_t(n) + 1), hipMemcpyHostToDevice, c.stream));
+    ADM_HIP(hipEventRecord(c.ev_h2d1, c.stream));
     return ADM_OK;
 }
 
@@ -541,6 +555,7 @@ int Engine::begin_collect(Context& c, int32_t n, std::string& err) {
         err = "collect: more strings than the last scan";
         return ADM_ERR_ARG;
     }
+    ADM_HIP(hipEventRecord(c.ev_d2h0, c.stream));
     ADM_HIP(hipMemcpyAsync(c.h_ctr, c.d_ctr, kCtrWords * sizeof(int32_t), hipMemcpyDeviceToHost, c.stream));
     if (n > 0) {
         ADM_HIP(hipMemcpyAsync(c.h_res_count, c.d_res_count, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c.stream));
@@ -566,10 +581,23 @@ int Engine::finish_collect(Context& c, int32_t n, std::vector<int32_t>& out_off,
         return ADM_ERR_POOL;
     }
     std::vector<uint32_t> pool(size_t(std::min<unsigned long long>(top, (unsigned long long)c.pool_cap)));
-    if (!pool.empty()) {
+    if (!pool.empty())
         ADM_HIP(hipMemcpyAsync(pool.data(), c.d_pool, sizeof(uint32_t) * pool.size(), hipMemcpyDeviceToHost,
                                c.stream));
-        ADM_HIP(hipStreamSynchronize(c.stream));
+    ADM_HIP(hipEventRecord(c.ev_d2h1, c.stream));
+    ADM_HIP(hipStreamSynchronize(c.stream));
+    if (hipEventElapsedTime(&ms, c.ev_d2h0, c.ev_d2h1) == hipSuccess) c.stats.d2h_ms = ms;
+    const auto tm = std::chrono::steady_clock::now();
+    if (c.emit_all) {
+        // every final state, ascending (the reference's get_active_states,
+        // src/state_vector.cpp:108-126)
+        c.fin_off.assign(size_t(n) + 1, 0);
+        c.fin_states.clear();
+        for (int32_t i = 0; i < n; ++i) {
+            const uint32_t* p = pool.data() + c.h_res_start[i];
+            c.fin_states.insert(c.fin_states.end(), p, p + c.h_res_count[i]);
+            c.fin_off[size_t(i) + 1] = int32_t(c.fin_states.size());
+        }
     }
     // accept map lookup, sort, unique (src/iNFA_kernel.cu:290-307)
     out_off.assign(size_t(n) + 1, 0);
@@ -586,6 +614,7 @@ int Engine::finish_collect(Context& c, int32_t n, std::vector<int32_t>& out_off,
         out_ids.insert(out_ids.end(), tmp.begin(), tmp.end());
         out_off[size_t(i) + 1] = int32_t(out_ids.size());
     }
+    c.stats.map_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - tm).count();
     return ADM_OK;
 }
 
@@ -596,19 +625,26 @@ int Engine::collect(Context& c, int32_t n, std::vector<int32_t>& out_off, std::v
 }
 
 int Engine::scan_host(const uint8_t* const* text, const int32_t* len, int32_t n, KernelKind kind, int32_t mode,
-                      std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err) {
+                      std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err,
+                      bool final_states) {
     if (!dev_ready_) {
         err = "rule set not on the device (allocScratch first)";
         return ADM_ERR_STATE;
     }
+    ctx_.stats.alloc_ms = 0.f;
     if (int rc = stage(ctx_, text, len, n, err)) return rc;
+    float h2d = 0.f;
     // a pool overflow grows the pool to fit, so the second scan succeeds
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        if (int rc = scan_device(ctx_, ctx_.d_text, ctx_.d_off, n, kind, mode, false, err)) return rc;
-        const int rc = collect(ctx_, n, out_off, out_ids, err);
-        if (rc != ADM_ERR_POOL) return rc;
+    ctx_.emit_all = final_states;
+    int rc = ADM_ERR_POOL;
+    for (int attempt = 0; attempt < 2 && rc == ADM_ERR_POOL; ++attempt) {
+        rc = scan_device(ctx_, ctx_.d_text, ctx_.d_off, n, kind, mode, false, err);
+        if (rc) break;
+        rc = collect(ctx_, n, out_off, out_ids, err);
     }
-    return ADM_ERR_POOL;
+    ctx_.emit_all = false;
+    if (rc == ADM_OK && hipEventElapsedTime(&h2d, ctx_.ev_h2d0, ctx_.ev_h2d1) == hipSuccess) ctx_.stats.h2d_ms = h2d;
+    return rc;
 }
 
 int Engine::set_mlp(const float* w, int32_t n, int32_t every, std::string& err) {

@@ -45,6 +45,11 @@ struct Variant {
 
 struct ScanStats {
     float kernel_ms = 0.f;  // last scan: hipEvent time of the launch sequence
+    // phases of the last host-strings scan (the reference's PROFILER_MODE,
+    // src/kernel_common.cu:57-95): device buffer growth (host clock; buffers
+    // are reused, so usually 0), host-to-device copies and result copies
+    // (hipEvents), accept-map lookup on the host (host clock)
+    float alloc_ms = 0.f, h2d_ms = 0.f, d2h_ms = 0.f, map_ms = 0.f;
     unsigned long long method_steps[3] = {0, 0, 0};
     long long pool_used = 0;
 };
@@ -64,6 +69,12 @@ struct StreamSet {
 struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
+    hipEvent_t ev_h2d0 = nullptr, ev_h2d1 = nullptr, ev_d2h0 = nullptr, ev_d2h1 = nullptr;
+    // emit every final state, not only the accepting ones; the last scan's
+    // final states of string i are fin_states[fin_off[i] .. fin_off[i+1])
+    bool emit_all = false;
+    std::vector<int32_t> fin_off;
+    std::vector<uint32_t> fin_states;
     int32_t res_n_cap = 0;
     int64_t pool_cap = 0;
     int32_t* d_res_count = nullptr;
@@ -126,8 +137,10 @@ class Engine {
     int collect(int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err) {
         return collect(ctx_, n, out_off, out_ids, err);
     }
+    // final_states: also keep every string's final states (ctx_.fin_*)
     int scan_host(const uint8_t* const* text, const int32_t* len, int32_t n, KernelKind kind, int32_t mode,
-                  std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err);
+                  std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids, std::string& err,
+                  bool final_states = false);
     const ScanStats& stats() const { return ctx_.stats; }
     Context& default_context() { return ctx_; }
 

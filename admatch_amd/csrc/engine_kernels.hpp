@@ -178,12 +178,14 @@ __device__ int rebuild_list(const uint64_t* bits, int W, uint32_t* list) {
     return n;
 }
 
-// Writes the accepted states (cur & accept, ascending) of string sid.
-static __device__ void emit_accepts(const DevGraph& g, const ScanArgs& a, int sid, const uint64_t* cur) {
+// Writes the accepted states (cur & accept, ascending) of string sid, or
+// every state of cur when a.emit_all.
+static inline __device__ void emit_accepts(const DevGraph& g, const ScanArgs& a, int sid, const uint64_t* cur) {
+    const uint64_t keep = a.emit_all ? ~0ull : 0ull;
     int total = 0;
     for (int j0 = 0; j0 < g.W; j0 += 64) {
         const int j = j0 + lane_id();
-        const uint64_t v = j < g.W ? (cur[j] & g.accept[j]) : 0ull;
+        const uint64_t v = j < g.W ? (cur[j] & (g.accept[j] | keep)) : 0ull;
         total += wave_sum(__popcll(v));
     }
     total = uniform(total);
@@ -199,7 +201,7 @@ static __device__ void emit_accepts(const DevGraph& g, const ScanArgs& a, int si
             int n = 0;
             for (int j0 = 0; j0 < g.W; j0 += 64) {
                 const int j = j0 + lane_id();
-                uint64_t v = j < g.W ? (cur[j] & g.accept[j]) : 0ull;
+                uint64_t v = j < g.W ? (cur[j] & (g.accept[j] | keep)) : 0ull;
                 const int cnt = __popcll(v);
                 const int incl = wave_scan(cnt);
                 int64_t pos = start + n + incl - cnt;
@@ -264,7 +266,7 @@ __device__ int scatter(const uint2* edges, int cnt, const uint64_t* cur, uint64_
 // TKO shift groups of symbol c: fut[j] |= shift(lim_i & cur, d_i).  Lane j
 // owns word j, so no atomics.  Shifts are split as d = 64q + r with
 // 0 <= r < 64 (floor division), and the r == 0 case never shifts by 64.
-static __device__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64_t* fut) {
+static inline __device__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64_t* fut) {
     const int W = g.W;
     const int kb = g.tko_koff[c], ke = g.tko_koff[c + 1];
     for (int i = kb; i < ke; ++i) {
@@ -472,7 +474,7 @@ __device__ __forceinline__ float feature(const DevGraph& g, const StepCtx& s, in
 // [64 in][64 out], so every weight load is one coalesced 256-byte row.
 // Returns argmin of the three predicted log step costs among the allowed
 // methods.
-static __device__ int mlp_choice(const float* w, float fl, bool allow_as) {
+static inline __device__ int mlp_choice(const float* w, float fl, bool allow_as) {
     const int j = lane_id();
     float h = w[kMlpB1 + j];
 #pragma unroll 4
@@ -972,14 +974,14 @@ __device__ __forceinline__ void dense_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-static __device__ int count_bits(const uint64_t* bits, int W) {
+static inline __device__ int count_bits(const uint64_t* bits, int W) {
     int n = 0;
     for (int j = lane_id(); j < W; j += 64) n += __popcll(bits[j]);
     return uniform(wave_sum(n));
 }
 
 // iNFA or TKO step on whole bitmaps (fut zero on entry).
-static __device__ void dense_method(const DevGraph& g, int m, int c, const uint64_t* cur, uint64_t* fut) {
+static inline __device__ void dense_method(const DevGraph& g, int m, int c, const uint64_t* cur, uint64_t* fut) {
     if (m == M_TKO) {
         tko_groups(g, c, cur, fut);
         dense_sync();
@@ -1242,7 +1244,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
     // held ones (not in the table), compacted into the free list; up to 64
     // are emitted by rank without touching the bitmaps
     int cnt = -1;
-    if (sparse) {
+    if (sparse && !a.emit_all) {
         cnt = 0;
         for (int k0 = 0; k0 < n_act; k0 += 64) {
             const int k = k0 + lane_id();
@@ -1299,7 +1301,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         if (sparse) {
             for (int k = lane_id(); k < n_act; k += 64) {
                 const uint32_t id = cl[k] & kIdMask;
-                if (test_bit(g.accept, id)) set_bit_atomic(gfut, id);
+                if (a.emit_all || test_bit(g.accept, id)) set_bit_atomic(gfut, id);
             }
             dense_sync();
             emit_accepts(g, a, sid, gfut);

@@ -35,9 +35,10 @@ void batched(ita_handle* h, char** text, int* text_len, int n, vector<int>* out)
     int rc = ita_batched_scan(h, reinterpret_cast<const uint8_t* const*>(text), text_len, n, off.data(), ids.data(),
                               int64_t(ids.size()));
     if (rc == ITA_ERR_CAPACITY) {
+        // the scan's results are kept by the handle: fetch them, no rescan
         ids.resize(size_t(off[n]));
-        rc = ita_batched_scan(h, reinterpret_cast<const uint8_t* const*>(text), text_len, n, off.data(), ids.data(),
-                              int64_t(ids.size()));
+        if (ita_last_results(h, ids.data(), int64_t(ids.size())) != int64_t(off[n])) die("BatchedScan");
+        rc = ITA_OK;
     }
     if (rc != ITA_OK) die("BatchedScan");
     // append, then sort + unique the whole vector (src/iNFA_kernel.cu:298-306)

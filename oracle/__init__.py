@@ -14,7 +14,6 @@ from __future__ import annotations
 
 import ctypes
 import os
-from collections import defaultdict
 
 import numpy as np
 
@@ -161,22 +160,21 @@ def tko_partition(nfa: OracleNFA, max_k: int = 1024):
             leftover[c] = t
             continue
         if pstates.size:
+            # add the self-loops, then sort + unique by (src, dst) as one key
             t = np.concatenate([t, np.stack([pstates, pstates], axis=1)])
-            t = np.unique(t, axis=0)
+            key = np.unique(t[:, 0] * np.int64(nfa.S) + t[:, 1])
+            t = np.stack([key // nfa.S, key % nfa.S], axis=1)
         d = t[:, 1] - t[:, 0]
-        by = defaultdict(list)
-        for i, dd in enumerate(d.tolist()):
-            by[dd].append(i)
-        ranked = sorted(by.items(), key=lambda kv: (-len(kv[1]), kv[0]))
-        sel, rest = [], []
-        for dd, idx in ranked:
-            if len(idx) > thr and len(sel) < max_k:
-                sel.append((dd, np.sort(t[idx, 0])))
-            else:
-                rest.extend(idx)
+        # groups by offset, ranked by size (largest first), ties by offset
+        offs, inv, cnt = np.unique(d, return_inverse=True, return_counts=True)
+        ranked = np.lexsort((offs, -cnt))
+        take = ranked[:max_k][cnt[ranked[:max_k]] > thr]
+        by_group = np.argsort(inv, kind="stable")  # t is sorted by source, so each group's sources stay sorted
+        starts = np.concatenate([[0], np.cumsum(cnt)])
+        sel = [(int(offs[g]), t[by_group[starts[g]:starts[g + 1]], 0]) for g in take]
         k_per[c] = len(sel)
         groups[c] = sel
-        lo = t[sorted(rest)] if rest else np.zeros((0, 2), np.int64)
+        lo = t[~np.isin(inv, take)]
         if lo.size:
             order = np.lexsort((lo[:, 1], lo[:, 0]))
             lo = lo[order]

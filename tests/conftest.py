@@ -38,6 +38,18 @@ def golden_json(name: str):
         return json.load(f)
 
 
+def ref_fixture_files(tmp_dir: str, subset: str):
+    """Paths of the per-rule .nfa files of data/<subset> (crs or snort),
+    extracted once from tests/golden/ref_nfa_fixtures.tar.gz."""
+    import tarfile
+    root = os.path.join(tmp_dir, "ref_nfa")
+    if not os.path.isdir(root):
+        with tarfile.open(os.path.join(GOLDEN, "ref_nfa_fixtures.tar.gz")) as t:
+            t.extractall(root, filter="data")
+    d = os.path.join(root, subset)
+    return sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(".nfa"))
+
+
 @pytest.fixture(scope="session")
 def gold_dir(tmp_path_factory):
     return str(tmp_path_factory.mktemp("golden"))

@@ -19,6 +19,8 @@ Outputs (all data, no reference source):
   synth_small_inputs.npz  256 of its inputs (seed 101) with planted matches
   synth_small_expected.json  rule ids from Python re.search on each rule's
                           regex source (independent of the oracle)
+  ref_nfa_fixtures.tar.gz every per-rule file of data/crs and data/snort as
+                          shipped (parser tests, anchored-semantics scans)
 """
 from __future__ import annotations
 
@@ -26,6 +28,7 @@ import gzip
 import json
 import os
 import re
+import subprocess
 import sys
 
 import numpy as np
@@ -95,6 +98,11 @@ def main(ref: str):
     exp = [[k + 1 for k, p in enumerate(pats) if p.search(s)] for s in strs]
     with open(os.path.join(HERE, "synth_small_expected.json"), "w") as f:
         json.dump({"rules": len(pats), "expected": exp}, f, separators=(",", ":"))
+
+    # reproducible archive of the per-rule files (fixed owner and mtime)
+    subprocess.run(["tar", "czf", os.path.join(HERE, "ref_nfa_fixtures.tar.gz"), "--sort=name", "--owner=0",
+                    "--group=0", "--numeric-owner", "--mtime=2020-01-01", "-C", os.path.join(ref, "data"), "crs",
+                    "snort"], check=True)
 
 
 def synth_pack(strings):

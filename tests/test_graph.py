@@ -5,7 +5,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import golden_nfa
+from conftest import golden_nfa, ref_fixture_files
 from nfa_gen import random_nfa, shift_nfa
 
 import oracle  # checker
@@ -56,16 +56,18 @@ def check_tko(s, o):
     loff = s.table(5)
     left = s.table(6).reshape(-1, 2)
     assert s.info()["tko_threshold"] == o.S // 16
-    for c in range(257):
-        gs = groups.get(c, [])
-        for j, (d, srcs) in enumerate(gs):
-            i = koff[c] + j
-            assert shifts[i] == d, (c, j)
-            bits = np.unpackbits(lim[i].view(np.uint8), bitorder="little")
-            assert (np.nonzero(bits)[0] == srcs).all(), (c, d)
-        lo = left[loff[c]:loff[c + 1]].astype(np.int64)
-        exp = leftover[c]
-        assert lo.shape[0] == exp.shape[0] and (lo == exp).all(), c
+    # groups in symbol order, ranked within a symbol: offsets and source masks
+    flat = [g for c in range(257) for g in groups.get(c, [])]
+    assert shifts.tolist() == [d for d, _ in flat]
+    rows = np.concatenate([np.full(len(src), i, np.int64) for i, (_, src) in enumerate(flat)] or [np.zeros(0, np.int64)])
+    srcs = np.concatenate([src for _, src in flat] or [np.zeros(0, np.int64)]).astype(np.int64)
+    want = np.zeros((len(flat), W), np.uint64)
+    np.bitwise_or.at(want, (rows, srcs >> 6), np.left_shift(np.uint64(1), (srcs & 63).astype(np.uint64)))
+    assert (lim == want).all()
+    # leftover lists per symbol
+    assert (np.diff(loff) == [leftover[c].shape[0] for c in range(257)]).all()
+    exp = np.concatenate([leftover[c] for c in range(257)]).astype(np.int64)
+    assert (left.astype(np.int64) == exp).all()
 
 
 @pytest.mark.parametrize("name", ["crs_g6.nfa.gz", "crs_g7.nfa.gz", "synth_small.nfa.gz"])
@@ -74,11 +76,32 @@ def test_tko_partition_fixtures(gold_dir, name):
     check_tko(s, o)
 
 
-def test_tko_partition_negative_offsets(gold_dir):
-    # group 7 of script/run.sh has top-k groups with negative offsets
-    s, o = same_as_oracle(golden_nfa("crs_g7.nfa.gz", gold_dir))
+@pytest.mark.parametrize("name,k_max,k_sum,neg,neg_unaligned", [
+    ("crs_g0.nfa.gz", 1, 256, 0, 0), ("crs_g6.nfa.gz", 5, 546, 0, 0),
+    ("crs_g7.nfa.gz", 48, 11992, 11992, 11737), ("crs_g9.nfa.gz", 69, 17595, 17595, 17085)])
+def test_tko_partition_group_figures(gold_dir, name, k_max, k_sum, neg, neg_unaligned):
+    # the top-k figures of the script/run.sh groups counted from the merged
+    # files by text parsing: largest k per symbol, total groups, groups with
+    # a negative offset, and those of them not a multiple of 32 (where the
+    # reference's unsigned promotion shifts by d - 32)
+    s, o = same_as_oracle(golden_nfa(name, gold_dir))
+    k = np.diff(s.table(2))
     shifts = s.table(3)
-    assert (shifts < 0).sum() > 1000 and ((shifts % 32) != 0).sum() > 1000
+    assert (int(k.max()), int(k.sum())) == (k_max, k_sum)
+    assert int((shifts < 0).sum()) == neg and int(((shifts < 0) & (shifts % 32 != 0)).sum()) == neg_unaligned
+    check_tko(s, o)
+
+
+@pytest.mark.parametrize("subset,count", [("crs", 214), ("snort", 405)])
+def test_every_fixture_file_matches_oracle(gold_dir, subset, count):
+    # the product's parse, iNFA lists, vectors and TKO tables equal the
+    # oracle's on every per-rule file the reference ships
+    files = ref_fixture_files(gold_dir, subset)
+    assert len(files) == count
+    for p in files:
+        s, o = same_as_oracle(p)
+        check_tko(s, o)
+        s.close()
 
 
 @pytest.mark.parametrize("seed", range(3))

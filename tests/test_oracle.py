@@ -6,7 +6,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import golden_inputs, golden_json, golden_nfa
+from conftest import golden_inputs, golden_json, golden_nfa, ref_fixture_files
 from nfa_gen import brute_force, burst_nfa, burst_strings, random_nfa, shift_nfa
 
 import oracle  # checker under test
@@ -131,12 +131,15 @@ def test_threads_do_not_change_results(workloads):
     assert a == b
 
 
-@pytest.mark.skipif(not os.path.isdir(REF), reason="reference checkout not mounted")
-def test_fixture_statistics():
-    # per-set totals measured from the raw files (data/crs, data/snort)
-    def totals(pattern):
+def test_fixture_statistics(gold_dir):
+    # per-set totals of the raw per-rule files (data/crs, data/snort), as
+    # counted by text parsing: 18,266 states, 1,875 word-boundary
+    # transitions, 867 persistent and 767 accepting states in crs; 405
+    # files, 34,443 states, 538 accepting states and no boundary or
+    # persistent state in snort
+    def totals(files):
         t = dict(states=0, wb=0, persist=0, accept=0, files=0)
-        for f in sorted(glob.glob(pattern)):
+        for f in files:
             o = oracle.OracleNFA(f)
             t["states"] += o.S
             t["wb"] += o.n_wb
@@ -144,7 +147,7 @@ def test_fixture_statistics():
             t["accept"] += o.n_accept_map
             t["files"] += 1
         return t
-    crs = totals(os.path.join(REF, "data", "crs", "*.nfa"))
-    snort = totals(os.path.join(REF, "data", "snort", "*.nfa"))
-    assert crs["states"] == 18266 and crs["persist"] == 867
-    assert snort["files"] == 405 and snort["states"] == 34443 and snort["wb"] == 0 and snort["persist"] == 0
+    crs = totals(ref_fixture_files(gold_dir, "crs"))
+    snort = totals(ref_fixture_files(gold_dir, "snort"))
+    assert (crs["states"], crs["wb"], crs["persist"], crs["accept"]) == (18266, 1875, 867, 767)
+    assert (snort["files"], snort["states"], snort["wb"], snort["persist"], snort["accept"]) == (405, 34443, 0, 0, 538)
