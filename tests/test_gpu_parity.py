@@ -164,3 +164,93 @@ def test_large_full_size(workloads):
     sbuf, soff = E.pack_strings([w.inputs.string(i) for i in sample])
     exp = oracle.OracleNFA(path).batch_scan(sbuf, soff, nthreads=os.cpu_count() or 4)
     assert [ref[i] for i in sample] == exp
+
+
+def _nfa_edges(path):
+    """(initial, accepting, {src: [(dst, byte list)]}) of a per-rule file."""
+    init, acc, edges = set(), set(), {}
+    for line in open(path, newline="").read().replace("\r", "").split("\n"):
+        if ":" not in line or line.startswith("#"):
+            continue
+        lhs, rhs = (x.strip() for x in line.split(":", 1))
+        if "->" in lhs:
+            a, b = (int(x) for x in lhs.split("->"))
+            syms = []
+            for atom in rhs.split():
+                lo, _, hi = atom.partition("|")
+                syms += list(range(int(lo), int(hi or lo) + 1))
+            edges.setdefault(a, []).append((b, [c for c in syms if c < 256]))
+        elif "initial" in rhs:
+            init.add(int(lhs))
+        elif "accepting" in rhs:
+            acc.add(int(lhs))
+    return init, acc, edges
+
+
+def _accepted_string(path, rng, max_len=300):
+    """A random member of the file's language: a path from an initial to an
+    accepting state (it must end there: acceptance is tested on the final
+    states only, and these files have no self-loop on their initial state)."""
+    init, acc, edges = _nfa_edges(path)
+    live = set(acc)  # states that can still reach an accepting state
+    grow = True
+    while grow:
+        grow = False
+        for a, outs in edges.items():
+            if a not in live and any(b in live and syms for b, syms in outs):
+                live.add(a)
+                grow = True
+    starts = sorted(init & live)
+    if not starts:
+        return None
+    for _ in range(50):
+        st, out = int(rng.choice(starts)), []
+        while len(out) < max_len:
+            if st in acc and (rng.random() < 0.3 or not edges.get(st)):
+                return bytes(out)
+            outs = [(b, syms) for b, syms in edges.get(st, []) if b in live and syms]
+            if not outs:
+                break
+            b, syms = outs[int(rng.integers(len(outs)))]
+            out.append(int(rng.choice(syms)))
+            st = b
+    return None
+
+
+def test_snort_anchored_rules(tmp_path, gold_dir):
+    # data/snort rules have no 0|255 self-loops: a rule accepts only a whole
+    # string of its language (anchored both ends).  A merged sample of them,
+    # scanned on every path, against the oracle; strings built from
+    # accepting paths must report their own rule, and the same strings with
+    # a byte added or dropped give the anchored answer the oracle computes
+    from admatch_amd import merge
+    from conftest import ref_fixture_files
+    files = ref_fixture_files(gold_dir, "snort")
+    rng = np.random.default_rng(11)
+    pick = sorted(rng.choice(len(files), size=80, replace=False).tolist())
+    chosen = sorted((files[i] for i in pick), key=lambda p: merge.natural_key(os.path.basename(p)))
+    path = str(tmp_path / "snort80.nfa")
+    with open(path, "w") as f:
+        f.write(merge.merge_files(chosen))
+    strs, own = [], []
+    for rid, p in enumerate(chosen, start=1):
+        for _ in range(3):
+            s = _accepted_string(p, rng)
+            if s is None:
+                continue
+            strs += [s, s + b"x", s[1:], b"GET " + s]
+            own += [rid, 0, 0, 0]
+    w = golden_inputs("http_inputs.npz")
+    strs += [w[0][w[1][i]:w[1][i + 1]].tobytes() for i in range(0, 200, 4)]
+    own += [0] * 50
+    o = oracle.OracleNFA(path)
+    exp = [o.scan(s) for s in strs]
+    hits = [i for i, r in enumerate(own) if r]
+    assert len(hits) > 100 and all(own[i] in exp[i] for i in hits)
+    buf, off = E.pack_strings(strs)
+    for name, flags in PATHS:
+        print(f"[case] snort80 {name} n={len(strs)}", flush=True)
+        got = scan_all(flags, path, buf, off)
+        assert got == exp, (name, first_diff(got, exp))
+    got = scan_all(E.AD_MATCHER, path, buf, off, variant=(2, 256, 64, 1))
+    assert got == exp, ("hashed", first_diff(got, exp))
