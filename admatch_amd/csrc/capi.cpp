@@ -193,6 +193,20 @@ int ita_scan_device(ita_handle* h, const uint8_t* d_text, const int64_t* d_offse
     return ITA_OK;
 }
 
+int ita_scan_device_noalloc(ita_handle* h, const uint8_t* d_text, const int64_t* d_offsets, int32_t n, int32_t kind,
+                            int32_t mode, void* stream, int32_t* d_counts, int64_t* d_starts, uint32_t* d_states,
+                            int64_t states_cap, int64_t* d_status) {
+    if (int rc = check(h)) return rc;
+    if (kind != ITA_KIND_PLAIN && kind != ITA_KIND_ENGINE) return fail(ITA_ERR_ARG, "bad kind");
+    if (mode < ITA_MODE_INFA || mode > ITA_MODE_LEARNED) return fail(ITA_ERR_ARG, "bad mode");
+    std::string err;
+    if (int rc = h->eng.scan_noalloc(d_text, d_offsets, n, admatch::KernelKind(kind), mode,
+                                     static_cast<hipStream_t>(stream), d_counts, d_starts, d_states, states_cap,
+                                     d_status, err))
+        return fail(rc, err);
+    return ITA_OK;
+}
+
 int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids, int64_t cap) {
     if (int rc = check(h)) return rc;
     if (n < 0 || !out_offsets) return fail(ITA_ERR_ARG, "bad output buffers");
@@ -365,6 +379,8 @@ int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap) {
         case 18: return give(g.wbs_dst.data(), int64_t(g.wbs_dst.size()), 4);
         case 19: return give(g.wbi_off.data(), int64_t(g.wbi_off.size()), 4);
         case 20: return give(g.wbi_dst.data(), int64_t(g.wbi_dst.size()), 4);
+        case 21: return give(g.acc_off.data(), int64_t(g.acc_off.size()), 4);
+        case 22: return give(g.acc_rules.data(), int64_t(g.acc_rules.size()), 4);
         default: return fail(ITA_ERR_ARG, "bad table id");
     }
 }

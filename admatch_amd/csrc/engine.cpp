@@ -263,6 +263,8 @@ int Engine::upload(std::string& err) {
     int lds = 0;
     ADM_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     max_lds_ = size_t(lds);
+    // the allocation-free entry point's array on this device
+    if (int rc = noalloc_bind_device(nullptr, err)) return rc;
     auto put = [&](const void* src, size_t bytes, const void** dst) -> int {
         void* p = nullptr;
         const size_t nb = bytes ? bytes : 8;

@@ -173,6 +173,15 @@ class Engine {
     int stream_scan(StreamSet& ss, KernelKind kind, int32_t mode, const uint8_t* const* text, const int32_t* len,
                     const int32_t* ids, int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids,
                     std::string& err);
+    // allocation-free scan: the launch sequence of scan_device on the caller's
+    // stream, its counters and processing order in the file-scope device
+    // array of noalloc.hip, results (accepted states per string, as
+    // collected by ita_collect before the rule lookup) in caller buffers;
+    // status[0] = pool overflow, status[1] = accepted states needed.  LDS
+    // storage only; at most kNoAllocMaxStrings strings.
+    int scan_noalloc(const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
+                     hipStream_t st, int32_t* d_counts, int64_t* d_starts, uint32_t* d_states, int64_t states_cap,
+                     int64_t* d_status, std::string& err);
     // HBM bitmaps of the hashed storage for context c (no-op for LDS storage)
     int prepare_dense(Context& c, int32_t mode, std::string& err);
     // the engine kernel a public mode runs (the learned mode's depends on the selector)
@@ -228,5 +237,12 @@ constexpr int kCtrCursors = 44;   // kBins cursors
 constexpr int kCtrMethods = 84;   // 4 x u64
 constexpr int kCtrLogTop = 92;    // u64
 constexpr int kCtrWords = 96;
+
+// strings per call of the allocation-free scan (its order array holds one
+// int32 per string)
+constexpr int32_t kNoAllocMaxStrings = 1 << 20;
+// looks up (once per device) the current device's copy of the
+// allocation-free scan's array
+int noalloc_bind_device(uint8_t** out, std::string& err);
 
 }  // namespace admatch

@@ -43,7 +43,7 @@ EXPORTED = [
     "ita_collect", "ita_last_stats", "ita_set_variant", "ita_get_variant", "ita_set_selector", "ita_set_log",
     "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
     "ita_selector_forward", "ita_serve_trace", "ita_stream_create", "ita_stream_reset", "ita_stream_scan",
-    "ita_stream_destroy", "ita_last_results",
+    "ita_stream_destroy", "ita_last_results", "ita_scan_device_noalloc",
 ]
 
 _LIB = None
@@ -71,6 +71,7 @@ def lib():
             "ita_batched_scan": (i32, [vp, vp, vp, i32, vp, vp, i64]),
             "ita_scan_device": (i32, [vp, vp, vp, i32, i32, i32, vp, i32]),
             "ita_collect": (i32, [vp, i32, vp, vp, i64]),
+            "ita_scan_device_noalloc": (i32, [vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i64, vp]),
             "ita_last_stats": (i32, [vp, vp, vp]),
             "ita_set_variant": (i32, [vp, i32, i32, i32, i32]),
             "ita_get_variant": (i32, [vp, vp, vp, vp, vp]),
@@ -177,6 +178,18 @@ class Scratch:
                     stream: Optional[int] = None, use_graph: bool = False):
         self._last_scan = (d_text, d_offsets, n, kind, mode, stream, int(use_graph))
         _err(lib().ita_scan_device(self._h, *self._last_scan))
+
+    def scan_device_noalloc(self, d_text: int, d_offsets: int, n: int, kind: int, mode: int, d_counts: int,
+                            d_starts: int, d_states: int, states_cap: int, d_status: int,
+                            stream: Optional[int] = None):
+        """ita_scan_device_noalloc: enqueues the scan, results in the given
+        device buffers (accepted states per string; see include/admatch.h)."""
+        _err(lib().ita_scan_device_noalloc(self._h, d_text, d_offsets, n, kind, mode, stream, d_counts, d_starts,
+                                           d_states, states_cap, d_status))
+
+    def accept_map(self) -> Tuple[np.ndarray, np.ndarray]:
+        """(offsets[S+1], rule ids): the rules of state s are ids[off[s]:off[s+1]]."""
+        return self.table(21), self.table(22)
 
     def collect(self, n: int) -> Tuple[np.ndarray, np.ndarray]:
         """Results of the last scan_device; a pool overflow (the library grows
@@ -367,6 +380,12 @@ class DeviceArray:
         self.ptr = p.value
         if host.nbytes:
             Hip.check(Hip.rt().hipMemcpy(self.ptr, host.ctypes.data, host.nbytes, 1), "hipMemcpy H2D")
+
+    def to_host(self, out: np.ndarray) -> np.ndarray:
+        """Copies the first out.nbytes bytes back into out (hipMemcpy D2H)."""
+        if out.nbytes:
+            Hip.check(Hip.rt().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2), "hipMemcpy D2H")
+        return out
 
     def free(self):
         if getattr(self, "ptr", None):

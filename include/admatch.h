@@ -101,6 +101,27 @@ int ita_batched_scan(ita_handle* h, const uint8_t* const* text, const int32_t* l
 int ita_scan_device(ita_handle* h, const uint8_t* d_text, const int64_t* d_offsets, int32_t n, int32_t kind,
                     int32_t mode, void* stream, int32_t use_graph);
 
+/* Allocation-free variant of ita_scan_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc, hipFree or other allocating routine
+ * and creates no stream: it enqueues the scan on `stream` (NULL: the null
+ * stream) and returns without waiting.  The kernels' counters and
+ * longest-first order live in a fixed device array of the library, about
+ * 4 MiB per GPU, held until the process ends; n may not exceed 2^20
+ * (ITA_ERR_TOO_LARGE).  Calls that overlap on one GPU share that array and
+ * race: serialise them.  kind/mode as ita_scan_device; the engine runs with
+ * LDS-bitmap storage whatever ita_set_variant's store says; the handle must
+ * be allocated (ita_alloc).  Outputs, all device memory of the caller: the
+ * accepted states of string i, ascending, in
+ * d_states[d_starts[i] .. d_starts[i] + d_counts[i]) (d_counts[i] < 0: the
+ * list did not fit in states_cap entries); d_status[0] = 1 if any list did
+ * not fit, d_status[1] = accepted states of all strings.  Rule ids of a
+ * state come from the accept map (ita_table 21 and 22). */
+int ita_scan_device_noalloc(ita_handle* h, const uint8_t* d_text, const int64_t* d_offsets, int32_t n, int32_t kind,
+                            int32_t mode, void* stream, int32_t* d_counts, int64_t* d_starts, uint32_t* d_states,
+                            int64_t states_cap, int64_t* d_status);
+
 /* Copy the last ita_scan_device results to the host and map accepted states
  * to rule ids (layout as ita_batched_scan). */
 int ita_collect(ita_handle* h, int32_t n, int32_t* out_offsets, int32_t* out_ids, int64_t cap);
@@ -179,7 +200,9 @@ int ita_info(ita_handle* h, int64_t* out11);
  * word-boundary successors of the implicit sets: offsets (258 int32, row c
  * for the always-active row of byte c, row 256 for the always-active states
  * alone), 20 their destinations (uint32); 17-20 are empty unless no
- * word-boundary destination is a word-boundary source.  Returns the element
+ * word-boundary destination is a word-boundary source; 21 accept map offsets
+ * (S+1 int32) and 22 its rule ids (int32): the rules of state s are
+ * [off[s], off[s+1]).  Returns the element
  * count; copies min(count, cap) elements when out != NULL. */
 int64_t ita_table(ita_handle* h, int32_t which, void* out, int64_t cap);
 
