@@ -2,7 +2,7 @@
 
 Sweeps the engine's state storage {LDS bitmaps, hashed lists with HBM
 bitmaps}, waves per workgroup {1, 2, 4} and active-list capacity
-{128, 256, 512, 1024}, and the plain iNFA kernel's workgroup size {64, 128, 256},
+{128, 256, 512, 1024}, and the plain iNFA kernel's workgroup size {64, ..., 1024},
 on a device-resident sample of the workload; keeps the fastest by hipEvent
 kernel time (all variants must give identical results).  Results are cached
 in admatch_amd/data/autotune.json keyed by the rule set's content hash.
@@ -60,7 +60,7 @@ def tune(s: E.Scratch, d_buf, d_off, n: int, mode: int = E.MODE_HEURISTIC, reps:
     plain_bs = 64
     if plain and info["plain_lds_bytes"] <= 160 * 1024:
         pr = []
-        for bs in (64, 128, 256):
+        for bs in (64, 128, 256, 512, 1024):
             s.set_variant(wpb, cap, bs, store)
             m = _time(s, d_buf, d_off, n, E.KIND_PLAIN, E.MODE_INFA, 1)
             log(f"  plain bs={bs}: {m:.3f} ms")

@@ -307,6 +307,12 @@ int ita_set_selector(ita_handle* h, const float* weights, int32_t n, int32_t eve
     return status(h->eng.set_mlp(weights, n, every, err), err);
 }
 
+int ita_set_policy(ita_handle* h, const uint32_t* table, int32_t n) {
+    if (int rc = check(h)) return rc;
+    std::string err;
+    return status(h->eng.set_policy(table, n, err), err);
+}
+
 int ita_set_log(ita_handle* h, int64_t cap_records, int32_t stride) {
     if (int rc = check(h)) return rc;
     std::string err;

@@ -16,7 +16,8 @@ struct DevGraph {
     const uint2* list;      // (src,dst) per symbol, sorted
     const int32_t* tko_koff;  // [258]
     const int32_t* tko_shift; // [K]
-    const uint64_t* tko_lim;  // [K*W]
+    const uint64_t* tko_lim;  // distinct source-mask rows, W words each
+    const int32_t* tko_row;   // [K] row of group i in tko_lim
     const int32_t* tko_loff;  // [258]
     const uint2* tko_left;    // leftover (src,dst)
     const uint32_t* as_tab;   // [256*S] AS rows, byte major (null: AS unavailable)

@@ -288,7 +288,36 @@ int Engine::upload(std::string& err) {
     ADM_PUT(g_.list, list)
     ADM_PUT(g_.tko_koff, tko_koff)
     ADM_PUT(g_.tko_shift, tko_shift)
-    ADM_PUT(g_.tko_lim, tko_lim)
+    {
+        // each distinct source-mask row once (groups of different bytes
+        // often share one: the persistent states, a class run's chain), so
+        // the rows a burst of steps reads stay in cache
+        const size_t W = size_t(g_.W);
+        const size_t K = g_.tko_shift.size();
+        std::vector<uint64_t> rows;
+        std::vector<int32_t> row_of(std::max<size_t>(K, 1), 0);
+        std::map<uint64_t, std::vector<int32_t>> by_hash;  // content hash -> rows
+        for (size_t i = 0; i < K; ++i) {
+            const uint64_t* r = g_.tko_lim.data() + i * W;
+            uint64_t h = 1469598103934665603ull;
+            for (size_t j = 0; j < W; ++j) h = (h ^ r[j]) * 1099511628211ull;
+            int32_t found = -1;
+            for (int32_t cand : by_hash[h])
+                if (std::equal(r, r + W, rows.data() + size_t(cand) * W)) {
+                    found = cand;
+                    break;
+                }
+            if (found < 0) {
+                found = int32_t(rows.size() / W);
+                rows.insert(rows.end(), r, r + W);
+                by_hash[h].push_back(found);
+            }
+            row_of[i] = found;
+        }
+        tko_rows_ = int64_t(rows.size() / std::max<size_t>(W, 1));
+        ADM_PUT(rows, tko_lim)
+        ADM_PUT(row_of, tko_row)
+    }
     ADM_PUT(g_.tko_loff, tko_loff)
     ADM_PUT(g_.tko_left, tko_left)
     if (g_.as_ok) {
@@ -448,7 +477,7 @@ int Engine::prepare_dense(Context& c, int32_t mode, std::string& err) {
 
 int Engine::engine_grid(int32_t mode, int32_t n, bool hs, int* grid, std::string& err) {
     const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
-    if (lds + sizeof(int4) * 256 > max_lds_) {
+    if (lds > max_lds_) {
         err = "rule set too large for the engine's LDS budget (lower the waves per workgroup)";
         return ADM_ERR_TOO_LARGE;
     }
@@ -762,6 +791,22 @@ int Engine::policy_table(uint32_t* out, std::string& err) {
         return ADM_ERR_STATE;
     }
     ADM_HIP(hipMemcpy(out, d_policy_, sizeof(uint32_t) * kPolicyTableWords, hipMemcpyDeviceToHost));
+    return ADM_OK;
+}
+
+int Engine::set_policy(const uint32_t* table, int32_t n, std::string& err) {
+    if (!d_mlp_ || !d_policy_) {
+        err = "no learned selector set";
+        return ADM_ERR_STATE;
+    }
+    if (!table || n != kPolicyTableWords) {
+        err = "policy table must hold " + std::to_string(kPolicyTableWords) + " words";
+        return ADM_ERR_ARG;
+    }
+    drop_all_graphs();
+    if (ctx_.stream) ADM_HIP(hipStreamSynchronize(ctx_.stream));
+    ADM_HIP(hipMemcpy(d_policy_, table, sizeof(uint32_t) * kPolicyTableWords, hipMemcpyHostToDevice));
+    h_policy_.assign(table, table + kPolicyTableWords);
     return ADM_OK;
 }
 

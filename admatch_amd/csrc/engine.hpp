@@ -190,6 +190,10 @@ class Engine {
     // the current list capacity lets a step reach, or -1
     int32_t policy_uniform(bool valid) const;
     int policy_table(uint32_t* out, std::string& err);  // kPolicyTableWords u32
+    // replaces the policy table the network was compiled into (a table the
+    // host derived from the network, e.g. its expected cost per entry over
+    // logged steps); needs a selector set
+    int set_policy(const uint32_t* table, int32_t n, std::string& err);
     bool has_mlp() const { return d_mlp_ != nullptr; }
     // step logging for the learned selector (MODE_LOG)
     int set_log(int64_t cap_records, int32_t stride, std::string& err);
@@ -201,6 +205,7 @@ class Engine {
     size_t plain_lds_bytes() const;
     int device_cu_count() const { return cu_count_; }
     size_t table_bytes() const { return table_bytes_; }
+    int64_t tko_distinct_rows() const { return tko_rows_; }  // after upload
 
    private:
     int engine_grid(int32_t mode, int32_t n, bool hashed_storage, int* grid, std::string& err);
@@ -214,6 +219,7 @@ class Engine {
     DevGraph dev_{};
     std::vector<void*> dev_allocs_;
     size_t table_bytes_ = 0;
+    int64_t tko_rows_ = 0;
     int cu_count_ = 0;
     size_t max_lds_ = 0;
     Variant var_;

@@ -239,19 +239,23 @@ __device__ int persist_sparse(const DevGraph& g, const uint32_t* cl, int n_cur, 
 }
 
 // Scatter step over (src,dst) edges: fut |= {dst : src in cur}.
-template <int CAP, bool TRACK>
+// U edges per lane, and then the U source words they test, are loaded
+// before any is used (memory-level parallelism: a round trip to L2 costs
+// hundreds of cycles).  Loads use a clamped index and are unconditional: a
+// guarded load compiles to a branch that waits for it before the next load
+// is issued; lanes past the end drop their result instead.
+template <int CAP, bool TRACK, int U = 4>
 __device__ int scatter(const uint2* edges, int cnt, const uint64_t* cur, uint64_t* fut, uint32_t* fl, int n) {
-    constexpr int U = 4;
     for (int t0 = 0; t0 < cnt; t0 += 64 * U) {
         uint2 e[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int t = t0 + u * 64 + lane_id();
-            e[u] = t < cnt ? edges[t] : make_uint2(0xffffffffu, 0u);
-        }
+        for (int u = 0; u < U; ++u) e[u] = edges[min(t0 + u * 64 + lane_id(), cnt - 1)];
+        uint64_t w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) w[u] = cur[e[u].x >> 6];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const bool hit = e[u].x != 0xffffffffu && test_bit(cur, e[u].x);
+            const bool hit = t0 + u * 64 + lane_id() < cnt && ((w[u] >> (e[u].x & 63)) & 1ull);
             if (TRACK) {
                 const bool nw = hit && set_new(fut, e[u].y);
                 n = list_append<CAP>(fl, n, nw, e[u].y);
@@ -263,24 +267,66 @@ __device__ int scatter(const uint2* edges, int cnt, const uint64_t* cur, uint64_
     return n;
 }
 
-// TKO shift groups of symbol c: fut[j] |= shift(lim_i & cur, d_i).  Lane j
-// owns word j, so no atomics.  Shifts are split as d = 64q + r with
-// 0 <= r < 64 (floor division), and the r == 0 case never shifts by 64.
-static inline __device__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64_t* fut) {
+// TKO shift groups of symbol c: fut[j] = OR over groups i of shift(lim_i &
+// cur, d_i)[j], fut zero on entry.  Lane l owns word j = j0 + 64u + l of U
+// consecutive 64-word chunks and accumulates every group in registers, so
+// each word is stored once and no atomics are needed; the U chunks' loads
+// are issued together.  Shifts are split as d = 64q + r with 0 <= r < 64
+// (floor division): word j takes source word j-q shifted left by r and,
+// when r != 0, source word j-q-1 shifted right by 64-r (the r == 0 case
+// never shifts by 64).  Word j-q-1 is the lane below's word j-q, taken with
+// a lane shuffle; lane 0 takes it from lane 63 of the previous chunk, or
+// loads it for the first chunk.
+template <int U>
+__device__ __forceinline__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64_t* fut) {
     const int W = g.W;
     const int kb = g.tko_koff[c], ke = g.tko_koff[c + 1];
-    for (int i = kb; i < ke; ++i) {
-        const int d = g.tko_shift[i];
-        const int q = d >> 6;  // arithmetic shift: floor(d / 64)
-        const int r = d & 63;
-        const uint64_t* lim = g.tko_lim + int64_t(i) * W;
-        for (int j = lane_id(); j < W; j += 64) {
-            const int s0 = j - q;
-            const int s1 = s0 - 1;
-            uint64_t v = 0;
-            if (s0 >= 0 && s0 < W) v = (lim[s0] & cur[s0]) << r;
-            if (r != 0 && s1 >= 0 && s1 < W) v |= (lim[s1] & cur[s1]) >> (64 - r);
-            if (v) fut[j] |= v;
+    if (kb == ke) return;
+    const int l = lane_id();
+    for (int j0 = 0; j0 < W; j0 += 64 * U) {
+        uint64_t acc[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc[u] = 0;
+        for (int i = kb; i < ke; ++i) {
+            const int d = g.tko_shift[i];
+            const int q = d >> 6;  // arithmetic shift: floor(d / 64)
+            const int r = d & 63;
+            const uint64_t* lim = g.tko_lim + int64_t(g.tko_row[i]) * W;
+            // clamped, unconditional loads (see scatter), issued together
+            uint64_t a[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int s0 = j0 + 64 * u + l - q;
+                const int sc = min(max(s0, 0), W - 1);
+                const uint64_t v = lim[sc] & cur[sc];
+                a[u] = s0 == sc ? v : 0ull;
+            }
+            if (r != 0) {
+                // the word below the first chunk (the same address in every lane)
+                const int s1 = j0 - q - 1;
+                const int sc1 = min(max(s1, 0), W - 1);
+                const uint64_t v1 = lim[sc1] & cur[sc1];
+                const uint64_t first = s1 == sc1 ? v1 : 0ull;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t up = uint64_t(__shfl_up(static_cast<long long>(a[u]), 1, 64));
+                    // lane 0: lane 63 of the previous chunk (a select, not a branch)
+                    const uint64_t fix = u == 0 ? first
+                                                : uint64_t(uint32_t(__builtin_amdgcn_readlane(int(a[u - 1] >> 32), 63)))
+                                                          << 32 |
+                                                      uint32_t(__builtin_amdgcn_readlane(int(a[u - 1]), 63));
+                    const uint64_t b = l == 0 ? fix : up;
+                    acc[u] |= (a[u] << r) | (b >> (64 - r));
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < U; ++u) acc[u] |= a[u];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + 64 * u + l;
+            if (j < W && acc[u]) fut[j] = acc[u];
         }
     }
 }
@@ -595,9 +641,12 @@ __device__ __forceinline__ int choose_method(const DevGraph& g, const ScanArgs& 
             if (m == M_AS && !sparse) m = heuristic_choice(g, ctx);
             break;
         case MODE_LOG:
+            // steps that are not timed run what a learned engine mostly runs
+            // (AS on a listed set, the heuristic's pick on a dense one), so
+            // the timed steps see the memory traffic of a real scan
             load_stats();
             ctx.n_act = total_active();
-            m = heuristic_choice(g, ctx);
+            m = sparse ? M_AS : heuristic_choice(g, ctx);
             log_step = ((uint64_t(sid) * 2654435761ull + uint64_t(b)) % uint64_t(a.log_stride)) == 0;
             break;
         default:
@@ -617,6 +666,24 @@ struct Lane {
     static constexpr size_t words(int W) { return size_t(2) * W + CAP; }
 };
 
+// fut = cur & persistent over all W words, U words per lane in flight.
+template <int U>
+__device__ __forceinline__ void persist_dense(const uint64_t* cur, const uint64_t* per, uint64_t* fut, int W) {
+    for (int j0 = lane_id(); j0 < W; j0 += 64 * U) {
+        uint64_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = min(j0 + 64 * u, W - 1);  // clamped, unconditional (see scatter)
+            v[u] = cur[j] & per[j];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = j0 + 64 * u;
+            if (j < W) fut[j] = v[u];
+        }
+    }
+}
+
 // Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
 // Returns the fut count (list valid iff *fut_sparse).
 template <int CAP>
@@ -625,26 +692,39 @@ __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, 
     int n = 0;
     const int W = g.W;
     if (m == M_TKO) {
-        tko_groups(g, c, cur, fut);
+        tko_groups<8>(g, c, cur, fut);
         wave_sync();
-        scatter<CAP, false>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, fl, 0);
+        scatter<CAP, false, 8>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, fl, 0);
         *fut_sparse = false;
         return -1;
     }
     if (sparse) {
         n = persist_sparse<CAP>(g, cl, n_cur, fut, fl);
     } else {
-        for (int j = lane_id(); j < W; j += 64) fut[j] = cur[j] & g.persis[j];
+        persist_dense<4>(cur, g.persis, fut, W);
     }
     wave_sync();
     if (sparse) {
-        n = scatter<CAP, true>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, n);
+        n = scatter<CAP, true, 8>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, n);
     } else {
-        scatter<CAP, false>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, 0);
+        scatter<CAP, false, 8>(g.list + g.sym_off[c], g.sym_off[c + 1] - g.sym_off[c], cur, fut, fl, 0);
         n = -1;
     }
     *fut_sparse = sparse && n <= CAP;
     return n;
+}
+
+// Set bits of bits[0..W) (whole wave), four words per lane in flight.
+static inline __device__ int count_bits(const uint64_t* bits, int W) {
+    int n = 0;
+    for (int j0 = lane_id(); j0 < W; j0 += 256) {
+        uint64_t v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = bits[min(j0 + 64 * u, W - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) n += j0 + 64 * u < W ? __popcll(v[u]) : 0;
+    }
+    return uniform(wave_sum(n));
 }
 
 template <int CAP>
@@ -868,7 +948,14 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 clear_bits<CAP>(fut, fl, n_fut, fsp, W);
                 wave_sync();
             }
+            // listing the implicitly held states is what leaving the AS
+            // form costs: charged to both other methods
+            __builtin_amdgcn_s_waitcnt(0);
+            const uint64_t tm0 = clock64();
             materialize();
+            wave_sync();
+            __builtin_amdgcn_s_waitcnt(0);
+            const float t_mat = float(clock64() - tm0);
             for (int mm = M_TKO; mm >= M_INFA; --mm) {
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
@@ -880,7 +967,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
                 }
                 wave_sync();
                 __builtin_amdgcn_s_waitcnt(0);
-                tm[mm] = float(clock64() - t0);
+                tm[mm] = float(clock64() - t0) + t_mat;
                 if (mm == M_TKO) {
                     clear_bits<CAP>(fut, fl, n_fut, fsp, W);
                     wave_sync();
@@ -922,8 +1009,12 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
             n_act = 0;
             sparse = false;
             materialize();
-            n_act = uniform(rebuild_list<CAP>(cur, W, cl));
-            sparse = n_act <= CAP;
+            // count first: a set that stays dense is not listed
+            n_act = count_bits(cur, W);
+            if (n_act <= CAP) {
+                n_act = uniform(rebuild_list<CAP>(cur, W, cl));
+                sparse = true;
+            }
         }
         // constant indices keep the counters in registers (AS steps: the
         // string length minus these)
@@ -974,16 +1065,11 @@ __device__ __forceinline__ void dense_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
-static inline __device__ int count_bits(const uint64_t* bits, int W) {
-    int n = 0;
-    for (int j = lane_id(); j < W; j += 64) n += __popcll(bits[j]);
-    return uniform(wave_sum(n));
-}
 
 // iNFA or TKO step on whole bitmaps (fut zero on entry).
 static inline __device__ void dense_method(const DevGraph& g, int m, int c, const uint64_t* cur, uint64_t* fut) {
     if (m == M_TKO) {
-        tko_groups(g, c, cur, fut);
+        tko_groups<1>(g, c, cur, fut);
         dense_sync();
         scatter<64, false>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, nullptr, 0);
     } else {
@@ -1218,7 +1304,9 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
             gcur = gfut;
             gfut = tb;
             dense_sync();
-            n_act = uniform(rebuild_list<CAP>(gcur, W, cl));
+            // count first: a set that stays dense is not listed
+            n_act = count_bits(gcur, W);
+            if (n_act <= CAP) n_act = uniform(rebuild_list<CAP>(gcur, W, cl));
             if (n_act <= CAP) {
                 // back to the list form
                 for (int k = lane_id(); k < n_act; k += 64) {
@@ -1372,12 +1460,9 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
 
 template <int WPB, int CAP, int MODE>
 __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a) {
+    // the per-byte statistics stay in device memory (scalar loads): a 4 KB
+    // copy per workgroup in LDS would cost large rule sets a workgroup per CU
     extern __shared__ uint64_t lds[];
-    // per-byte statistics shared by the workgroup's waves
-    __shared__ int4 s_stats[256];
-    for (int i = threadIdx.x; i < 256; i += 64 * WPB) s_stats[i] = g.sym_stats[i];
-    __syncthreads();
-    g.sym_stats = s_stats;
     const int wave = threadIdx.x >> 6;
     const size_t per = Lane<CAP>::words(g.W);
     uint64_t* base = lds + per * wave;

@@ -9,9 +9,8 @@ namespace admatch {
 
 constexpr int kLenBins = 40;
 
-// dynamic LDS the engine may request: the CU's 160 KB minus the static
-// per-workgroup byte statistics
-constexpr int kEngineDynLds = 160 * 1024 - int(sizeof(int4)) * 256;
+// dynamic LDS the engine may request: the CU's 160 KB
+constexpr int kEngineDynLds = 160 * 1024;
 
 hipError_t launch_engine(int wpb, int cap, int mode, bool hashed, int grid, size_t lds, hipStream_t st,
                          const DevGraph& g, const ScanArgs& a);

@@ -43,7 +43,7 @@ EXPORTED = [
     "ita_collect", "ita_last_stats", "ita_set_variant", "ita_get_variant", "ita_set_selector", "ita_set_log",
     "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
     "ita_selector_forward", "ita_serve_trace", "ita_stream_create", "ita_stream_reset", "ita_stream_scan",
-    "ita_stream_destroy", "ita_last_results", "ita_scan_device_noalloc",
+    "ita_stream_destroy", "ita_last_results", "ita_scan_device_noalloc", "ita_set_policy",
 ]
 
 _LIB = None
@@ -76,6 +76,7 @@ def lib():
             "ita_set_variant": (i32, [vp, i32, i32, i32, i32]),
             "ita_get_variant": (i32, [vp, vp, vp, vp, vp]),
             "ita_set_selector": (i32, [vp, vp, i32, i32]),
+            "ita_set_policy": (i32, [vp, vp, i32]),
             "ita_stream_create": (i32, [vp, i32, ctypes.POINTER(vp)]),
             "ita_stream_reset": (i32, [vp, i32]),
             "ita_stream_scan": (i32, [vp, vp, vp, vp, i32, vp, vp, i64]),
@@ -233,6 +234,11 @@ class Scratch:
             return
         w = np.ascontiguousarray(weights, dtype=np.float32)
         _err(lib().ita_set_selector(self._h, _p(w), w.size, every))
+
+    def set_policy(self, table: np.ndarray):
+        """Installs a policy table (512 u32) derived from the selector's network."""
+        t = np.ascontiguousarray(table, dtype=np.uint32)
+        _err(lib().ita_set_policy(self._h, _p(t), t.size))
 
     def policy_table(self) -> np.ndarray:
         """The learned selector's table (512 u32) of the current weights."""

@@ -221,6 +221,66 @@ def policy_lookup(table: np.ndarray, n_expl, sparse, c) -> np.ndarray:
     return ((table[e >> 4] >> ((e & 15) * 2).astype(np.uint32)) & 3).astype(np.int64)
 
 
+def policy_entries(ds: Dataset) -> np.ndarray:
+    """Table entry of each logged step."""
+    return (policy_bucket(ds.n_expl) * 2 + (ds.x[:, 10] > 0).astype(np.int64)) * 256 + ds.c.astype(np.int64)
+
+
+def policy_from_log(scratch: E.Scratch, w: np.ndarray, ds: Dataset) -> np.ndarray:
+    """The policy table of least expected cost under the network: for every
+    entry (count bucket, list validity, byte) that logged steps fall into,
+    the method whose predicted cycles, summed over those steps, are lowest
+    (AS only where the list is valid); the other entries keep the network's
+    choice at the synthetic step context (what ita_set_selector built).  The
+    table is what the network predicts for the steps the engine meets, where
+    the synthetic context fixes features (position, the method before) that
+    vary within an entry."""
+    scratch.set_selector(w)
+    base = scratch.policy_table()
+    e_all = np.arange(POLICY_BUCKETS * 2 * 256)
+    choice = ((base[e_all >> 4] >> ((e_all & 15) * 2).astype(np.uint32)) & 3).astype(np.int64)
+    cost = np.exp2(gpu_forward(ds.x, w).astype(np.float64))
+    ent = policy_entries(ds)
+    sums = np.zeros((e_all.size, O))
+    np.add.at(sums, ent, cost)
+    seen = np.bincount(ent, minlength=e_all.size) > 0
+    valid = ((e_all >> 8) & 1) == 1
+    sums[~valid, 2] = np.inf
+    choice[seen] = sums[seen].argmin(1)
+    return pack_policy(choice)
+
+
+# a table derived for one rule set, stored with that rule set's content hash
+POLICY_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector_policy.npz")
+
+
 def load_weights(path: Optional[str] = None) -> Optional[np.ndarray]:
     path = path or WEIGHTS_PATH
     return np.load(path) if os.path.exists(path) else None
+
+
+def save_policy(table: np.ndarray, rule_set_hash: int, path: Optional[str] = None) -> None:
+    np.savez(path or POLICY_PATH, table=np.asarray(table, dtype=np.uint32), hash=np.int64(rule_set_hash))
+
+
+def load_policy(rule_set_hash: int, path: Optional[str] = None) -> Optional[np.ndarray]:
+    """The stored table if it was derived for the rule set with this hash."""
+    path = path or POLICY_PATH
+    if not os.path.exists(path):
+        return None
+    z = np.load(path)
+    return z["table"] if int(z["hash"]) == int(rule_set_hash) else None
+
+
+def apply(scratch: E.Scratch, every: int = 0) -> bool:
+    """Loads the stored weights, and the policy table derived from them for
+    this rule set when one is stored, into the scratch; False when no
+    weights are stored."""
+    w = load_weights()
+    if w is None:
+        return False
+    scratch.set_selector(w, every=every)
+    t = load_policy(scratch.info()["hash"])
+    if t is not None and every == 0:
+        scratch.set_policy(t)
+    return True

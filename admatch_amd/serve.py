@@ -94,7 +94,7 @@ def main(argv=None):
     autotune.apply(s)
     wts = selector.load_weights()
     if wts is not None:
-        s.set_selector(wts)
+        selector.apply(s)
     report = {"workload": f"{w.ruleset.state_count} states, {w.inputs.n} requests, "
                           f"{int(w.inputs.offsets[-1])} B", "learned_selector": wts is not None, "rates": {}}
     for rate in (float(r) for r in args.rates.split(",")):

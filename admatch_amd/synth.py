@@ -241,6 +241,12 @@ class GrammarConfig:
     gap_rule_frac: float = 0.1   # fraction of rules containing a '.*' gap
     vocab_prefix_frac: float = 0.3  # literal runs starting with a traffic token
     pieces: Tuple[int, int] = (2, 4)
+    # dense shapes: gap rules open on a traffic token plus one byte (the '.*'
+    # right after it) and continue with a run of the burst class; a fraction
+    # of the other rules start with such a run
+    gap_after_token: bool = False
+    class_lead_frac: float = 0.0
+    gap_class_max: int = 8  # repetition bound of the class run after a gap
 
 
 def _rand_literal(rng, lo=4, hi=12, vocab_frac=0.0) -> bytes:
@@ -252,7 +258,29 @@ def _rand_literal(rng, lo=4, hi=12, vocab_frac=0.0) -> bytes:
     return bytes(ALPHABET[int(i)] for i in rng.integers(0, len(ALPHABET), size=n))
 
 
+def _dense_rule(rng, cfg: GrammarConfig, with_gap: bool):
+    burst = Sym(BURST_CLASS)
+    if with_gap:
+        # token + one more byte, '.*', a burst-class run, a literal: the gap
+        # opens on the (uncommon) pair, and from then on every burst-class
+        # byte re-enters the run
+        tok = VOCAB[int(rng.integers(len(VOCAB)))] + bytes([ALPHABET[int(rng.integers(36))]])
+        items = [_lit(tok), Star(Sym(ALL_BYTES)), _rep(burst, 1, cfg.gap_class_max),
+                 _lit(_rand_literal(rng), rng.random() < cfg.icase_frac)]
+        return Cat(tuple(items))
+    # a burst-class run, then literal pieces
+    items = [_rep(burst, 1, cfg.gap_class_max), _lit(_rand_literal(rng, 4, 8), rng.random() < cfg.icase_frac)]
+    for _ in range(int(rng.integers(cfg.pieces[0], cfg.pieces[1] + 1)) - 2):
+        items.append(_lit(_rand_literal(rng), rng.random() < cfg.icase_frac))
+    return Cat(tuple(items))
+
+
 def random_rule(rng, cfg: GrammarConfig, with_gap: bool):
+    # (no draw when there are no class-led rules: the other grammars keep
+    # their random streams)
+    if (with_gap and cfg.gap_after_token) or (not with_gap and cfg.class_lead_frac > 0
+                                              and rng.random() < cfg.class_lead_frac):
+        return _dense_rule(rng, cfg, with_gap)
     npieces = int(rng.integers(cfg.pieces[0], cfg.pieces[1] + 1))
     items = [_lit(_rand_literal(rng, vocab_frac=cfg.vocab_prefix_frac), rng.random() < cfg.icase_frac)]
     gap_at = int(rng.integers(1, npieces)) if with_gap and npieces >= 2 else -1
@@ -342,10 +370,20 @@ def sample_match(node, rng, max_gap: int = 24) -> bytes:
 _METHODS = [b"GET ", b"POST ", b"PUT ", b"HEAD "]
 
 
-def _corpus(rng, nbytes: int) -> np.ndarray:
-    """HTTP-request-like ASCII text: method, path, key=value query."""
+# Long runs of one class in traffic (session tokens, encoded blobs): the
+# byte class of the dense workload's bursts -- uppercase letters that the
+# request line's method and protocol names (GET, POST, PUT, HEAD, HTTP) do
+# not use, so ordinary requests never enter a burst.
+BURST_CLASS = frozenset(b"BCFIJKLMNQRVWXYZ")
+
+
+def _corpus(rng, nbytes: int, burst_frac: float = 0.0) -> np.ndarray:
+    """HTTP-request-like ASCII text: method, path, key=value query.  With
+    burst_frac > 0, that fraction of requests carries one more query value:
+    a token of 24-96 bytes of BURST_CLASS."""
     chunks, total = [], 0
     alpha = np.frombuffer(ALPHABET[:36], dtype=np.uint8)
+    burst = np.frombuffer(bytes(sorted(BURST_CLASS)), dtype=np.uint8)
     while total < nbytes:
         parts = [_METHODS[int(rng.integers(len(_METHODS)))], b"/"]
         for _ in range(int(rng.integers(1, 5))):
@@ -364,6 +402,8 @@ def _corpus(rng, nbytes: int) -> np.ndarray:
             if rng.random() < 0.15:
                 v += b"%2F" + alpha[rng.integers(0, 36, size=3)].tobytes()
             parts.append(v)
+        if burst_frac > 0 and rng.random() < burst_frac:
+            parts.append(b"&sid=" + burst[rng.integers(0, burst.size, size=int(rng.integers(24, 97)))].tobytes())
         parts.append(b" HTTP/1.1 ")
         s = b"".join(parts)
         chunks.append(s)
@@ -399,12 +439,12 @@ def make_lengths(rng, n: int, dist: str, lo: int = 200, hi: int = 1500,
 
 
 def make_inputs(rules: Optional[Sequence[object]], n: int, lengths: np.ndarray, planted_frac: float,
-                seed: int) -> Inputs:
+                seed: int, burst_frac: float = 0.0) -> Inputs:
     rng = np.random.default_rng(seed)
     lengths = np.asarray(lengths, dtype=np.int64)
     total = int(lengths.sum())
     # strings are windows of a shared request corpus (at most 4 MB of text)
-    corpus = _corpus(rng, min(total, 4 << 20) + int(lengths.max()) + 4096)
+    corpus = _corpus(rng, min(total, 4 << 20) + int(lengths.max()) + 4096, burst_frac)
     offsets = np.zeros(n + 1, dtype=np.int64)
     offsets[1:] = np.cumsum(lengths)
     starts = rng.integers(0, corpus.size - int(lengths.max()) - 1, size=n)
@@ -436,7 +476,8 @@ class Workload:
 
 
 # (rules seed, inputs seed) per workload, recorded here so runs are repeatable.
-SEEDS = {"small": (1, 101), "medium": (2, 102), "large": (3, 103), "train": (3, 104), "serve": (3, 105)}
+SEEDS = {"small": (1, 101), "medium": (2, 102), "large": (3, 103), "train": (3, 104), "serve": (3, 105),
+         "large_sparse": (3, 103)}
 
 
 def grammar_for(name: str) -> GrammarConfig:
@@ -444,9 +485,24 @@ def grammar_for(name: str) -> GrammarConfig:
         return GrammarConfig(n_rules=64, icase_frac=0.0, gap_rule_frac=0.15, pieces=(3, 4))
     if name == "medium":     # 1,000 rules, ~20k states, 20% case-insensitive literals
         return GrammarConfig(n_rules=1000, icase_frac=0.2, gap_rule_frac=0.15, pieces=(2, 3))
-    if name in ("large", "train", "serve"):  # 3,000 rules, ~80k states, 30% with '.*'
+    if name in ("large", "train", "serve"):
+        # 3,000 rules, ~70k states, 30% with '.*' gaps (opening on a traffic
+        # token plus one byte, then a burst-class run {1,8}), 45% of the rest
+        # led by a burst-class run {1,8}: a few dozen states are active in
+        # ordinary text, a few thousand on the uppercase method and protocol
+        # names, and more than 10% of the states inside long burst-class
+        # tokens (make_workload's burst_frac)
+        return GrammarConfig(n_rules=3000, icase_frac=0.2, gap_rule_frac=0.3, pieces=(2, 4), gap_after_token=True,
+                             class_lead_frac=0.45)
+    if name == "large_sparse":
+        # the same sizes without the dense shapes: the active set stays
+        # below 0.3% of the states
         return GrammarConfig(n_rules=3000, icase_frac=0.2, gap_rule_frac=0.3, pieces=(2, 4))
     raise ValueError(name)
+
+
+# Fraction of requests carrying a long burst-class token, per workload.
+BURST_FRAC = {"large": 0.1, "train": 0.1, "serve": 0.1}
 
 
 def make_workload(name: str, n: Optional[int] = None, seed_offset: int = 0) -> Workload:
@@ -469,9 +525,9 @@ def make_workload(name: str, n: Optional[int] = None, seed_offset: int = 0) -> W
         n = n or 65536
         lengths = make_lengths(rng, n, "lognormal", median=800, sigma=0.8, max_len=16384)
         frac = 0.002
-    else:
+    else:  # large, large_sparse
         n = n or 65536
         lengths = make_lengths(rng, n, "lognormal", median=1024, sigma=0.8, max_len=16384)
         frac = 0.002
-    inputs = make_inputs(rs.rules, n, lengths, frac, iseed + 7919 * seed_offset)
+    inputs = make_inputs(rs.rules, n, lengths, frac, iseed + 7919 * seed_offset, BURST_FRAC.get(name, 0.0))
     return Workload(name=name, ruleset=rs, inputs=inputs)

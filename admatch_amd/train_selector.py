@@ -6,12 +6,17 @@
    training strings (seed 104); the engine runs in logging mode and times
    all three step methods on one sampled step in `stride`;
 2. training: Adam on the GPU (ita_selector_train), log2-cycle targets;
-3. evaluation on held-out strings of the large workload (seed 103):
-   offline regret of the network, of the policy table the engine reads,
-   and of the heuristic on held-out logs, and
+3. the policy table the engine reads: per (count bucket, list validity,
+   byte) entry, the method of least predicted cost summed over the logged
+   training steps in the entry (the network's choice at a synthetic step
+   context where no step was logged);
+4. evaluation on held-out strings of the large workload (seed 103):
+   offline regret of the network, of the policy tables and of the
+   heuristic on held-out logs, and
    measured scan time of the learned selector vs the heuristic selectors,
    with bit-identical results required;
-4. writes the weights (.npy) and a JSON report beside them.
+5. writes the weights (.npy), the table (selector_policy.npz, with the rule
+   set's hash) and a JSON report beside them.
 """
 from __future__ import annotations
 
@@ -69,6 +74,7 @@ def main(argv=None):
 
     t0 = time.time()
     rec = selector.collect(s, *dtr, tr.inputs.n, cap=1 << 21, stride=args.stride)
+    ds_large = selector.from_log(rec)
     if args.extra_medium > 0:
         # a second rule set (1,000 rules, W = 356 words) so that the network
         # sees more than one state-vector size
@@ -94,10 +100,15 @@ def main(argv=None):
     pred = selector.gpu_forward(ds_ev.x, w)
     rl = selector.regret(ds_ev, selector.choose(pred, allow))
     rh = selector.regret(ds_ev, selector.heuristic_choice(ds_ev, W))
-    # what the engine reads: the network folded into the policy table
+    # what the engine reads: the network folded into the policy table, at
+    # synthetic step contexts and as the least expected cost over the
+    # training steps of this rule set
     s.set_selector(w, every=0)
     rt = selector.regret(ds_ev, selector.policy_lookup(s.policy_table(), ds_ev.n_expl, allow, ds_ev.c))
-    log(f"held-out steps: network {rl}, policy table {rt}, heuristic {rh}")
+    table = selector.policy_from_log(s, w, ds_large)
+    rd = selector.regret(ds_ev, selector.policy_lookup(table, ds_ev.n_expl, allow, ds_ev.c))
+    s.set_policy(table)
+    log(f"held-out steps: network {rl}, synthetic-context table {rt}, table from the log {rd}, heuristic {rh}")
 
     # measured scans on held-out strings
     timings, ref = {}, None
@@ -117,12 +128,14 @@ def main(argv=None):
 
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     np.save(args.out, w)
+    selector.save_policy(table, s.info()["hash"], path=os.path.join(os.path.dirname(args.out), "selector_policy.npz"))
     report = {
         "rule_set": {"states": tr.ruleset.state_count, "rules": len(tr.ruleset.rules), "hash": s.info()["hash"]},
         "train": {"strings": tr.inputs.n, "logged_steps": ds.n, "stride": args.stride, "epochs": args.epochs,
                   "batch": args.batch, "lr": args.lr, "loss_first": float(hist[0]), "loss_last": float(hist[-1]),
                   "best_method_counts": np.bincount(ds.best(), minlength=3).tolist()},
-        "heldout_steps": {"n": ds_ev.n, "network": rl, "policy_table": rt, "heuristic": rh},
+        "heldout_steps": {"n": ds_ev.n, "network": rl, "policy_table_synthetic_context": rt,
+                          "policy_table_from_log": rd, "heuristic": rh},
         "heldout_scan": {"strings": ev.inputs.n, "bytes": int(ev.inputs.offsets[-1]), **timings},
         "variant": dict(zip(("wpb", "cap", "plain_bs", "store"), s.variant())),
     }

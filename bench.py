@@ -232,7 +232,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="large", choices=["small", "medium", "large"])
+    ap.add_argument("--config", default="large", choices=["small", "medium", "large", "large_sparse"])
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="plain,admatch,heuristic,as,learned")
@@ -246,7 +246,7 @@ def main():
     ap.add_argument("--no-readme", dest="readme", action="store_false",
                     help="skip the reference README workload (CRS group 6 x bing.txt x 1024)")
     ap.add_argument("--variant", default=None,
-                    help="wpb,cap,store: engine launch variant (default: the autotuned one)")
+                    help="wpb,cap,store[,plain_bs]: launch variant (default: the autotuned one)")
     ap.add_argument("--tune", default=None, metavar="PATH",
                     help="autotune the launch variant on this workload first and store it in PATH")
     args = ap.parse_args()
@@ -278,18 +278,17 @@ def main():
             tuned = json.load(f).get(str(info["hash"]))
     if tuned:
         s.set_variant(tuned["wpb"], tuned["cap"], tuned.get("plain_bs", 64), tuned.get("store", 0))
-    learned_ok = False
-    if os.path.exists(SELECTOR_PATH):
-        s.set_selector(np.load(SELECTOR_PATH), every=0)
-        learned_ok = True
+    from admatch_amd import selector as SEL
+    learned_ok = SEL.apply(s)
     d_buf, d_off = E.DeviceArray(w.inputs.buf), E.DeviceArray(w.inputs.offsets)
     if args.tune:
         from admatch_amd import autotune
-        best = autotune.tune(s, d_buf, d_off, n, mode=E.MODE_AS, reps=1, log=log)
+        # tuned in the mode the headline runs
+        best = autotune.tune(s, d_buf, d_off, n, mode=E.MODE_LEARNED if learned_ok else E.MODE_AS, reps=1, log=log)
         autotune.store(info["hash"], best, path=args.tune)
     if args.variant:
-        wpb, cap, store = (int(x) for x in args.variant.split(","))
-        s.set_variant(wpb, cap, s.variant()[2], store)
+        v = [int(x) for x in args.variant.split(",")]
+        s.set_variant(v[0], v[1], v[3] if len(v) > 3 else s.variant()[2], v[2])
     log(f"variant (wpb, cap, plain_bs, store) = {s.variant()}")
 
     modes = {

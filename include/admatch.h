@@ -170,6 +170,14 @@ int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs
  * is used). */
 int ita_set_selector(ita_handle* h, const float* weights, int32_t n, int32_t every);
 
+/* Replaces the selector's policy table (table 16 of ita_table: n = 512
+ * words, layout there) with one the host derived from the network, such as
+ * the method of least predicted cost per entry summed over logged steps
+ * (admatch_amd/selector.py policy_from_log).  Needs ita_set_selector first;
+ * ita_set_selector rebuilds the table from the network again.  No
+ * counterpart in the reference. */
+int ita_set_policy(ita_handle* h, const uint32_t* table, int32_t n);
+
 /* Step logging for selector training (ITA_MODE_LOG): room for cap records
  * of 22 floats (16 features, cycles of iNFA, TKO and AS or -1, the
  * heuristic's method, the byte, the explicit active count), one step

@@ -122,7 +122,7 @@ def test_log_mode_and_learned_gain(workloads):
     # the logged total count includes the implicitly held states
     assert (np.rint(np.exp2(ds.x[:, 0]) - 1) >= ds.n_expl).all()
     half = ds.n // 2
-    tr = selector.Dataset(ds.x[:half], ds.y[:half], ds.mask[:half])
+    tr = selector.Dataset(ds.x[:half], ds.y[:half], ds.mask[:half], ds.c[:half], ds.n_expl[:half])
     te = selector.Dataset(ds.x[half:], ds.y[half:], ds.mask[half:], ds.c[half:], ds.n_expl[half:])
     wts, hist = selector.train(tr, epochs=20, batch=512, lr=2e-3, seed=0)
     assert hist[-1] < hist[0]
@@ -131,14 +131,18 @@ def test_log_mode_and_learned_gain(workloads):
     heur = selector.regret(te, selector.heuristic_choice(te, s.info()["words"]))
     # the network's choices cost fewer measured cycles than the heuristic's
     assert learned["picked_cycles"] < heur["picked_cycles"], (learned, heur)
-    # so do the policy table's, which the engine actually reads
-    s.set_selector(wts)
-    table = selector.policy_lookup(s.policy_table(), te.n_expl, te.x[:, 10] > 0, te.c)
-    tab = selector.regret(te, table)
+    # so do those of the policy table the engine reads (least predicted cost
+    # per entry over the training steps)
+    table = selector.policy_from_log(s, wts, tr)
+    s.set_policy(table)
+    assert np.array_equal(s.policy_table(), table)
+    tab = selector.regret(te, selector.policy_lookup(table, te.n_expl, te.x[:, 10] > 0, te.c))
     assert tab["picked_cycles"] < heur["picked_cycles"], (tab, heur)
-    # and the learned engine returns the same rule ids
+    # and the learned engine returns the same rule ids (table from the log,
+    # then the network in the scan every 4 steps)
     for every in (0, 4):
-        s.set_selector(wts, every=every)
+        if every:
+            s.set_selector(wts, every=every)
         s.scan_device(d_buf.ptr, d_off.ptr, n, E.KIND_ENGINE, E.MODE_LEARNED)
         got = s.collect(n)
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
