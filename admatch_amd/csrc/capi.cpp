@@ -410,6 +410,32 @@ int ita_selector_forward(const float* features, int64_t n, const float* weights,
     return status(admatch::mlp_forward(features, n, weights, out, err), err);
 }
 
+int ita_serve_ring(ita_handle* h, int32_t mode, const uint8_t* buf, const int64_t* offsets, const double* arrival_s,
+                   int32_t n, double* latency_s, int32_t* out_offsets, int32_t* out_ids, int64_t cap, double* stats6) {
+    if (int rc = check(h)) return rc;
+    if (mode < ITA_MODE_INFA || mode > ITA_MODE_LEARNED) return fail(ITA_ERR_ARG, "bad mode");
+    if (n < 0 || (n > 0 && (!latency_s || !out_offsets))) return fail(ITA_ERR_ARG, "null output");
+    admatch::Engine::RingResult r;
+    std::string err;
+    if (int rc = h->eng.serve_ring(mode, buf, offsets, arrival_s, n, r, err)) return fail(rc, err);
+    if (n > 0) {
+        std::memcpy(latency_s, r.latency_s.data(), sizeof(double) * size_t(n));
+        std::memcpy(out_offsets, r.offsets.data(), sizeof(int32_t) * (size_t(n) + 1));
+    }
+    if (stats6) {
+        stats6[0] = r.span_s;
+        stats6[1] = double(r.rescanned);
+        stats6[2] = double(r.waves);
+        for (int k = 0; k < 3; ++k) stats6[3 + k] = double(r.method_steps[k]);
+    }
+    const int64_t total = r.offsets.empty() ? 0 : r.offsets.back();
+    h->off = r.offsets;
+    h->ids = r.ids;
+    if (total > cap) return fail(ITA_ERR_CAPACITY, "out_ids too small (ita_last_results has them)");
+    if (total) std::memcpy(out_ids, r.ids.data(), sizeof(int32_t) * size_t(total));
+    return ITA_OK;
+}
+
 int ita_serve_trace(ita_handle* h, int32_t kind, int32_t mode, int32_t lanes, int32_t max_batch,
                     int64_t max_batch_bytes, int64_t state_budget_bytes, int64_t split_bytes, const uint8_t* buf,
                     const int64_t* offsets,

@@ -76,6 +76,18 @@ struct ScanArgs {
     uint8_t* stream_out_flags;
     int32_t policy_sparse, policy_dense;  // the table's method for every entry of that list validity, or -1
     int32_t emit_all;         // write every final state, not only the accepting ones (SHOW_RESULTS)
+    // device-side serving (Engine::serve_ring): ticket t may start once the
+    // host has published more than t requests.  Requests (text, offsets) live
+    // in host memory and are copied into the wave's staging slot; results
+    // (res_*) and a done flag per request go to host memory.
+    int32_t serve;
+    const int32_t* serve_published;  // host: requests published so far
+    const int32_t* serve_abort;      // host: nonzero stops every wait
+    int32_t* serve_mirror;           // device: [0] published as last read, [1] stop seen
+    int32_t* serve_done;             // host: per request, 1 once its results are written
+    uint8_t* serve_stage;            // device: serve_slot bytes per wave of the grid
+    int64_t serve_slot;
+    int64_t serve_wait_ticks;        // longest wait for one ticket, 100 MHz wall-clock ticks
 };
 
 enum SelectMode : int32_t {

@@ -462,7 +462,11 @@ int32_t Engine::kernel_mode(int32_t mode) const {
     if (mode != MODE_LEARNED) return mode;
     if (!d_mlp_) return MODE_HEUR;
     if (mlp_every_ > 0) return MODE_LEARNED_EVERY;
-    return policy_uniform(true) == M_AS ? MODE_LEARNED_AS : MODE_LEARNED;
+    if (policy_uniform(true) != M_AS) return MODE_LEARNED;
+    // AS on every listed set and iNFA on every bitmap-only one is exactly
+    // the AS mode's rule: run its kernel (fewer registers than the kernel
+    // that reads the table for bitmap-only steps)
+    return policy_uniform(false) == M_INFA ? MODE_AS : MODE_LEARNED_AS;
 }
 
 int Engine::prepare_dense(Context& c, int32_t mode, std::string& err) {

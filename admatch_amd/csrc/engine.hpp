@@ -182,6 +182,20 @@ class Engine {
     int scan_noalloc(const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
                      hipStream_t st, int32_t* d_counts, int64_t* d_starts, uint32_t* d_states, int64_t states_cap,
                      int64_t* d_status, std::string& err);
+    // Device-side serving: replays n requests (buf[offsets[i] .. offsets[i+1]),
+    // arriving arrival_s[i] seconds after the start, non-decreasing) through
+    // one persistent engine launch whose waves take each request as soon as
+    // the host publishes its arrival (serve.cpp).
+    struct RingResult {
+        std::vector<double> latency_s;     // completion seen by the host - arrival
+        std::vector<int32_t> offsets, ids;  // rule ids per request
+        double span_s = 0;                  // first arrival to last completion
+        int64_t rescanned = 0;              // requests rescanned after an output overflow
+        int32_t waves = 0;                  // waves of the persistent grid
+        unsigned long long method_steps[3] = {0, 0, 0};
+    };
+    int serve_ring(int32_t mode, const uint8_t* buf, const int64_t* offsets, const double* arrival_s, int32_t n,
+                   RingResult& out, std::string& err);
     // HBM bitmaps of the hashed storage for context c (no-op for LDS storage)
     int prepare_dense(Context& c, int32_t mode, std::string& err);
     // the engine kernel a public mode runs (the learned mode's depends on the selector)
@@ -242,6 +256,7 @@ constexpr int kCtrBins = 4;       // kBins counts
 constexpr int kCtrCursors = 44;   // kBins cursors
 constexpr int kCtrMethods = 84;   // 4 x u64
 constexpr int kCtrLogTop = 92;    // u64
+constexpr int kCtrServe = 94;     // serving: published count as last read, stop flag
 constexpr int kCtrWords = 96;
 
 // strings per call of the allocation-free scan (its order array holds one

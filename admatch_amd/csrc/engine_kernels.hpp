@@ -737,11 +737,9 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
 }
 
 template <int CAP, int MODE>
-__device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* bitsA, uint64_t* bitsB,
-                            uint32_t* listA, uint32_t* listB, int* msteps) {
+__device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const uint8_t* x, int L, uint64_t* bitsA,
+                            uint64_t* bitsB, uint32_t* listA, uint32_t* listB, int* msteps) {
     const int W = g.W;
-    const int64_t beg = a.offsets[sid];
-    const int L = int(a.offsets[sid + 1] - beg);
     msteps[3] += L;
     uint64_t* cur = bitsA;
     uint64_t* fut = bitsB;
@@ -760,7 +758,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     wave_sync();
     int n_act = uniform(rebuild_list<CAP>(cur, W, cl));
     bool sparse = n_act <= CAP;
-    ByteWindow win{a.text + beg, L, 0, 0u};
+    ByteWindow win{x, L, 0, 0u};
     if (L > 0) win.load(0);
     const bool has_wb = g.wb_count > 0;
     const bool have_pair = g.pair_off != nullptr;
@@ -1048,6 +1046,91 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, uint6
     wave_sync();
 }
 
+// ------------------------------------------------------------ serving
+// Device-side admission (ScanArgs::serve): each wave takes the next ticket
+// and waits until the host has published that many requests.  Only the wave
+// holding the lowest unpublished ticket reads the host's counter (over PCIe)
+// and mirrors what it saw into device memory; the others poll the mirror.
+// Every wait is bounded: tickets past the trace's end exit at once, a stop
+// flag (the host's abort, or a wait that outlived serve_wait_ticks) ends
+// every wait, so every wave exits.
+
+// Lane 0 polls; true when request t may start.
+__device__ bool serve_wait(const ScanArgs& a, int t) {
+    if (t >= a.n) return false;
+    int go = 0;
+    if (lane_id() == 0) {
+        const uint64_t t0 = wall_clock64();
+        for (;;) {
+            const int m = __hip_atomic_load(&a.serve_mirror[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            if (m > t) {
+                go = 1;
+                break;
+            }
+            if (__hip_atomic_load(&a.serve_mirror[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+            if (m == t) {
+                // the lowest unpublished ticket: read the host's counter
+                const int p = __hip_atomic_load(a.serve_published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (p > t) {
+                    atomicMax(&a.serve_mirror[0], p);
+                    go = 1;
+                    break;
+                }
+                if (__hip_atomic_load(a.serve_abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0) {
+                    atomicOr(&a.serve_mirror[1], 1);
+                    break;
+                }
+            }
+            if (wall_clock64() - t0 > uint64_t(a.serve_wait_ticks)) {
+                atomicOr(&a.serve_mirror[1], 2);
+                break;
+            }
+            // the head polls often; waves further back sleep longer (all
+            // waiting waves read one mirror word, so their reads are spread)
+            if (m == t) {
+                __builtin_amdgcn_s_sleep(2);
+            } else if (t - m < 256) {
+                __builtin_amdgcn_s_sleep(32);
+            } else {
+                __builtin_amdgcn_s_sleep(127);
+                __builtin_amdgcn_s_sleep(127);
+                __builtin_amdgcn_s_sleep(127);
+                __builtin_amdgcn_s_sleep(127);
+            }
+        }
+    }
+    return __builtin_amdgcn_readlane(go, 0) != 0;
+}
+
+// Copies request bytes [beg, beg+L) of host memory into the wave's staging
+// slot with 16-byte loads from the aligned-down start (the host buffer is
+// 16-byte aligned and padded past its end); returns where the string starts
+// in the slot.
+__device__ const uint8_t* serve_fetch(const ScanArgs& a, int64_t beg, int L, uint8_t* slot) {
+    const int64_t b0 = beg & ~int64_t(15);
+    const int nvec = int((beg + L - b0 + 15) >> 4);
+    const uint4* src = reinterpret_cast<const uint4*>(a.text + b0);
+    uint4* dst = reinterpret_cast<uint4*>(slot);
+    for (int k0 = 0; k0 < nvec; k0 += 256) {
+        uint4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = src[min(k0 + 64 * u + lane_id(), nvec - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k0 + 64 * u + lane_id() < nvec) dst[k0 + 64 * u + lane_id()] = v[u];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    return slot + (beg - b0);
+}
+
+// The request's results (written to host memory by the scan) before its
+// done flag.
+__device__ void serve_finish(const ScanArgs& a, int sid) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+    if (lane_id() == 0) __hip_atomic_store(&a.serve_done[sid], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 // ------------------------------------------------------------ hashed storage
 // The same step loop with the set held as the active list plus an LDS hash
 // table of 2*CAP slots per list (instead of two S-bit bitmaps), so a wave's
@@ -1081,16 +1164,15 @@ static inline __device__ void dense_method(const DevGraph& g, int m, int c, cons
 }
 
 template <int CAP, int MODE>
-__device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uint64_t* gA, uint64_t* gB,
-                              uint32_t* hA, uint32_t* hB, uint32_t* listA, uint32_t* listB, int* msteps) {
+__device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, const uint8_t* x, int L, uint64_t* gA,
+                              uint64_t* gB, uint32_t* hA, uint32_t* hB, uint32_t* listA, uint32_t* listB,
+                              int* msteps) {
     using HS = HashSet<2 * CAP>;
     // the selector mode, and whether the boundary code and the path for
     // missing two-byte rows are compiled in
     constexpr int M = MODE & ~kModeLean;
     constexpr bool kLean = (MODE & kModeLean) != 0;
     const int W = g.W;
-    const int64_t beg = a.offsets[sid];
-    const int L = int(a.offsets[sid + 1] - beg);
     msteps[3] += L;
     uint64_t* gcur = gA;
     uint64_t* gfut = gB;
@@ -1107,7 +1189,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, uin
         dense_sync();
     }
     wave_sync();
-    ByteWindow win{a.text + beg, L, 0, 0u};
+    ByteWindow win{x, L, 0, 0u};
     if (L > 0) win.load(0);
     const bool has_wb = !kLean && g.wb_count > 0;
     const bool have_pair = kLean || g.pair_off != nullptr;
@@ -1444,13 +1526,22 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
     HashSet<2 * CAP>{hB}.init_sink();
     wave_sync();
     int msteps[4] = {0, 0, 0, 0};  // iNFA, TKO, AS (filled below), bytes scanned
+    uint8_t* stage = a.serve_stage + (size_t(blockIdx.x) * WPB + wave) * size_t(a.serve_slot);
     for (;;) {
         int idx = 0;
         if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
-        if (idx >= a.n) break;
+        if (a.serve) {
+            if (!serve_wait(a, idx)) break;
+        } else if (idx >= a.n) {
+            break;
+        }
         const int sid = a.order ? a.order[idx] : idx;
-        scan_string_h<CAP, MODE>(g, a, sid, gA, gB, hA, hB, listA, listB, msteps);
+        const int64_t beg = a.offsets[sid];
+        const int L = int(a.offsets[sid + 1] - beg);
+        const uint8_t* x = a.serve ? serve_fetch(a, beg, L, stage) : a.text + beg;
+        scan_string_h<CAP, MODE>(g, a, sid, x, L, gA, gB, hA, hB, listA, listB, msteps);
+        if (a.serve) serve_finish(a, sid);
     }
     msteps[2] = msteps[3] - msteps[0] - msteps[1];
     if (a.method_steps != nullptr && lane_id() == 0)
@@ -1473,13 +1564,22 @@ __global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a
     for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
     wave_sync();
     int msteps[4] = {0, 0, 0, 0};  // iNFA, TKO, AS (filled below), bytes scanned
+    uint8_t* stage = a.serve_stage + (size_t(blockIdx.x) * WPB + wave) * size_t(a.serve_slot);
     for (;;) {
         int idx = 0;
         if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
-        if (idx >= a.n) break;
+        if (a.serve) {
+            if (!serve_wait(a, idx)) break;
+        } else if (idx >= a.n) {
+            break;
+        }
         const int sid = a.order ? a.order[idx] : idx;
-        scan_string<CAP, MODE>(g, a, sid, bitsA, bitsB, listA, listB, msteps);
+        const int64_t beg = a.offsets[sid];
+        const int L = int(a.offsets[sid + 1] - beg);
+        const uint8_t* x = a.serve ? serve_fetch(a, beg, L, stage) : a.text + beg;
+        scan_string<CAP, MODE>(g, a, sid, x, L, bitsA, bitsB, listA, listB, msteps);
+        if (a.serve) serve_finish(a, sid);
     }
     msteps[2] = msteps[3] - msteps[0] - msteps[1];
     if (a.method_steps != nullptr && lane_id() == 0)

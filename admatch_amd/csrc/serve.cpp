@@ -16,10 +16,14 @@
 // latency is its completion time minus its arrival time.
 #include "serve.hpp"
 
+#include "launch.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <deque>
+#include <limits>
+#include <memory>
 #include <thread>
 
 namespace admatch {
@@ -199,6 +203,231 @@ int serve_trace(Engine& eng, const ServeConfig& cfg, const uint8_t* buf, const i
     out.wall_s = now_s();
     out.mean_batch = out.batches ? batch_sum / double(out.batches) : 0.0;
     out.offsets.assign(size_t(n) + 1, 0);
+    out.ids.clear();
+    for (int32_t i = 0; i < n; ++i) {
+        out.ids.insert(out.ids.end(), res[size_t(i)].begin(), res[size_t(i)].end());
+        out.offsets[size_t(i) + 1] = int32_t(out.ids.size());
+    }
+    return ADM_OK;
+}
+
+}  // namespace admatch
+
+// ------------------------------------------------------------ device-side serving
+namespace admatch {
+
+namespace {
+// pinned host memory the GPU reads and writes directly
+struct Pinned {
+    void* h = nullptr;
+    void* d = nullptr;
+    ~Pinned() {
+        if (h) (void)hipHostFree(h);
+    }
+    hipError_t alloc(size_t bytes) {
+        hipError_t e = hipHostMalloc(&h, bytes ? bytes : 8, hipHostMallocMapped | hipHostMallocCoherent);
+        if (e != hipSuccess) return e;
+        std::memset(h, 0, bytes ? bytes : 8);
+        return hipHostGetDevicePointer(&d, h, 0);
+    }
+};
+
+#define ADM_RING_HIP(call)                                                  \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) {                                             \
+            err = std::string(#call) + ": " + hipGetErrorString(e_);        \
+            return ADM_ERR_HIP;                                             \
+        }                                                                   \
+    } while (0)
+}  // namespace
+
+int Engine::serve_ring(int32_t mode, const uint8_t* buf, const int64_t* offsets, const double* arrival_s, int32_t n,
+                       RingResult& out, std::string& err) {
+    if (!dev_ready_) {
+        err = "rule set not on the device (allocScratch first)";
+        return ADM_ERR_STATE;
+    }
+    if (n < 0 || (n > 0 && (!buf || !offsets || !arrival_s))) {
+        err = "bad arguments";
+        return ADM_ERR_ARG;
+    }
+    for (int32_t i = 1; i < n; ++i)
+        if (arrival_s[i] < arrival_s[i - 1]) {
+            err = "arrival times must be non-decreasing";
+            return ADM_ERR_ARG;
+        }
+    int64_t max_len = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        const int64_t len = offsets[i + 1] - offsets[i];
+        if (len < 0 || len > (int64_t(1) << 30)) {
+            err = "bad request length";
+            return ADM_ERR_ARG;
+        }
+        max_len = std::max(max_len, len);
+    }
+    out = RingResult();
+    out.latency_s.assign(size_t(n), 0.0);
+    out.offsets.assign(size_t(n) + 1, 0);
+    if (n == 0) return ADM_OK;
+    const int64_t total = offsets[n] - offsets[0];
+    mode = kernel_mode(mode);
+    const bool hs = hashed(var_, mode);
+    std::unique_ptr<Context> cp;
+    if (int rc = new_context(cp, err)) return rc;
+    Context& c = *cp;
+    struct Forget {
+        Engine& e;
+        Context* c;
+        ~Forget() { e.forget_context(c); }
+    } forget{*this, cp.get()};
+    if (hs)
+        if (int rc = prepare_dense(c, mode, err)) return rc;
+    int grid = 0;
+    if (int rc = engine_grid(mode, std::numeric_limits<int32_t>::max(), hs, &grid, err)) return rc;
+    const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
+    const int64_t slot = (max_len + 32 + 15) / 16 * 16;
+    const int64_t waves = int64_t(grid) * var_.wpb;
+    uint8_t* d_stage = nullptr;
+    ADM_RING_HIP(hipMalloc(&d_stage, size_t(waves * slot)));
+    struct FreeStage {
+        uint8_t* p;
+        ~FreeStage() { (void)hipFree(p); }
+    } free_stage{d_stage};
+
+    // requests, as a network stack would have left them: in host memory
+    Pinned text, offs, flags, done, cnt, start, pool;
+    const int64_t pool_cap = std::max<int64_t>(int64_t(8) * n, int64_t(1) << 16);
+    ADM_RING_HIP(text.alloc(size_t(total) + 64));
+    ADM_RING_HIP(offs.alloc(sizeof(int64_t) * (size_t(n) + 1)));
+    ADM_RING_HIP(flags.alloc(64));
+    ADM_RING_HIP(done.alloc(sizeof(int32_t) * size_t(n)));
+    ADM_RING_HIP(cnt.alloc(sizeof(int32_t) * size_t(n)));
+    ADM_RING_HIP(start.alloc(sizeof(int64_t) * size_t(n)));
+    ADM_RING_HIP(pool.alloc(sizeof(uint32_t) * size_t(pool_cap)));
+    std::memcpy(text.h, buf + offsets[0], size_t(total));
+    int64_t* h_off = static_cast<int64_t*>(offs.h);
+    for (int32_t i = 0; i <= n; ++i) h_off[i] = offsets[i] - offsets[0];
+    int32_t* h_flags = static_cast<int32_t*>(flags.h);  // [0] published, [1] abort
+    int32_t* h_done = static_cast<int32_t*>(done.h);
+
+    ScanArgs a = make_args(c, static_cast<const uint8_t*>(text.d), static_cast<const int64_t*>(offs.d), n, mode);
+    a.order = nullptr;
+    a.res_count = static_cast<int32_t*>(cnt.d);
+    a.res_start = static_cast<int64_t*>(start.d);
+    a.res_pool = static_cast<uint32_t*>(pool.d);
+    a.pool_cap = pool_cap;
+    a.serve = 1;
+    a.serve_published = static_cast<const int32_t*>(flags.d);
+    a.serve_abort = static_cast<const int32_t*>(flags.d) + 1;
+    a.serve_mirror = c.d_ctr + kCtrServe;
+    a.serve_done = static_cast<int32_t*>(done.d);
+    a.serve_stage = d_stage;
+    a.serve_slot = slot;
+    // a ticket waits at most until the last arrival plus 10 s (100 MHz ticks)
+    a.serve_wait_ticks = int64_t((arrival_s[n - 1] - arrival_s[0] + 10.0) * 1e8);
+    if (hs) a.dense = c.d_dense;
+    const int32_t kmode = (hs && g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
+    ADM_RING_HIP(launch_fill(c.d_ctr, 0, kCtrWords, c.stream));
+    ADM_RING_HIP(launch_engine(var_.wpb, var_.cap, kmode, hs, grid, lds, c.stream, dev_, a));
+    out.waves = int32_t(waves);
+
+    // replay: publish each request at its arrival, watch the done flags
+    auto stop_kernel = [&]() {
+        __atomic_store_n(&h_flags[1], 1, __ATOMIC_RELEASE);
+        (void)hipStreamSynchronize(c.stream);
+    };
+    std::vector<uint8_t> seen(size_t(n), 0);
+    const auto t0 = clk::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(clk::now() - t0).count() + arrival_s[0]; };
+    int32_t published = 0, completed = 0, lo = 0;
+    double last_done = arrival_s[0];
+    while (completed < n) {
+        const double t = now_s();
+        int32_t p = published;
+        while (p < n && arrival_s[p] <= t) ++p;
+        if (p != published) {
+            published = p;
+            __atomic_store_n(&h_flags[0], published, __ATOMIC_RELEASE);
+        }
+        bool progressed = false;
+        for (int32_t i = lo; i < published; ++i) {
+            if (seen[size_t(i)] || !__atomic_load_n(&h_done[i], __ATOMIC_ACQUIRE)) continue;
+            seen[size_t(i)] = 1;
+            last_done = now_s();
+            out.latency_s[size_t(i)] = last_done - arrival_s[i];
+            ++completed;
+            progressed = true;
+        }
+        while (lo < n && seen[size_t(lo)]) ++lo;
+        if (progressed) continue;
+        if (t > arrival_s[n - 1] + 60.0) {
+            err = "serving: requests still open a minute after the last arrival";
+            stop_kernel();
+            return ADM_ERR_HIP;
+        }
+        const hipError_t q = hipStreamQuery(c.stream);
+        if (q == hipSuccess) {
+            err = "serving kernel ended before every request completed";
+            stop_kernel();
+            return ADM_ERR_HIP;
+        }
+        if (q != hipErrorNotReady) {
+            err = std::string("serving kernel: ") + hipGetErrorString(q);
+            stop_kernel();
+            return ADM_ERR_HIP;
+        }
+        if (published < n && completed == published) {
+            const double wait = arrival_s[published] - now_s();
+            if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
+        }
+    }
+    out.span_s = last_done - arrival_s[0];
+    // every ticket past the last request exits: the kernel ends
+    ADM_RING_HIP(hipStreamSynchronize(c.stream));
+    ADM_RING_HIP(hipMemcpy(c.h_ctr, c.d_ctr, kCtrWords * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (c.h_ctr[kCtrServe + 1] != 0) {
+        err = "serving kernel stopped a wait (host abort or wait limit)";
+        return ADM_ERR_HIP;
+    }
+    std::memcpy(out.method_steps, c.h_ctr + kCtrMethods, sizeof(out.method_steps));
+
+    // accepted states -> sorted unique rule ids (src/iNFA_kernel.cu:290-307);
+    // a request whose states did not fit the pool is scanned again
+    const int32_t* h_cnt = static_cast<const int32_t*>(cnt.h);
+    const int64_t* h_start = static_cast<const int64_t*>(start.h);
+    const uint32_t* h_pool = static_cast<const uint32_t*>(pool.h);
+    std::vector<std::vector<int32_t>> res(static_cast<size_t>(n));
+    std::vector<int32_t> again;
+    std::vector<int32_t> tmp;
+    for (int32_t i = 0; i < n; ++i) {
+        if (h_cnt[i] < 0) {
+            again.push_back(i);
+            continue;
+        }
+        tmp.clear();
+        for (int32_t k = 0; k < h_cnt[i]; ++k) {
+            const uint32_t s = h_pool[h_start[i] + k];
+            for (int32_t r = g_.acc_off[s]; r < g_.acc_off[s + 1]; ++r) tmp.push_back(g_.acc_rules[r]);
+        }
+        std::sort(tmp.begin(), tmp.end());
+        tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+        res[size_t(i)] = tmp;
+    }
+    if (!again.empty()) {
+        std::vector<const uint8_t*> ptr;
+        std::vector<int32_t> len;
+        for (int32_t i : again) {
+            ptr.push_back(buf + offsets[i]);
+            len.push_back(int32_t(offsets[i + 1] - offsets[i]));
+        }
+        std::vector<int32_t> off2, ids2;
+        if (int rc = scan_host(ptr.data(), len.data(), int32_t(again.size()), KIND_ENGINE, mode, off2, ids2, err))
+            return rc;
+        for (size_t k = 0; k < again.size(); ++k)
+            res[size_t(again[k])].assign(ids2.begin() + off2[k], ids2.begin() + off2[k + 1]);
+        out.rescanned = int64_t(again.size());
+    }
     out.ids.clear();
     for (int32_t i = 0; i < n; ++i) {
         out.ids.insert(out.ids.end(), res[size_t(i)].begin(), res[size_t(i)].end());

@@ -43,7 +43,7 @@ EXPORTED = [
     "ita_collect", "ita_last_stats", "ita_set_variant", "ita_get_variant", "ita_set_selector", "ita_set_log",
     "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
     "ita_selector_forward", "ita_serve_trace", "ita_stream_create", "ita_stream_reset", "ita_stream_scan",
-    "ita_stream_destroy", "ita_last_results", "ita_scan_device_noalloc", "ita_set_policy",
+    "ita_stream_destroy", "ita_last_results", "ita_scan_device_noalloc", "ita_set_policy", "ita_serve_ring",
 ]
 
 _LIB = None

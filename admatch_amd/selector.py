@@ -272,6 +272,54 @@ def load_policy(rule_set_hash: int, path: Optional[str] = None) -> Optional[np.n
     return z["table"] if int(z["hash"]) == int(rule_set_hash) else None
 
 
+def as_on_listed(table: np.ndarray) -> np.ndarray:
+    """The table with every list-valid entry set to AS (the network still
+    decides the steps on bitmap-only sets)."""
+    e = np.arange(POLICY_BUCKETS * 2 * 256)
+    choice = ((table[e >> 4] >> ((e & 15) * 2).astype(np.uint32)) & 3).astype(np.int64)
+    choice[((e >> 8) & 1) == 1] = 2
+    return pack_policy(choice)
+
+
+def as_rule_table() -> np.ndarray:
+    """AS on every listed set, iNFA on every bitmap-only one: the fixed rule
+    of the AS mode, as a table."""
+    e = np.arange(POLICY_BUCKETS * 2 * 256)
+    return pack_policy(np.where(((e >> 8) & 1) == 1, 2, 0))
+
+
+def calibrate(scratch: E.Scratch, buf: np.ndarray, offsets: np.ndarray, stride: int = 4,
+              reps: int = 3) -> Optional[np.ndarray]:
+    """Compiles the stored network into a policy table for this rule set and
+    its traffic, on a sample of strings drawn apart from the ones to be
+    scanned: one logging scan, then the table of least predicted cost over
+    the logged steps (policy_from_log).  The predictions are per-step
+    latencies; whether they also win for the whole scan is checked by
+    timing that table against the same table with AS on every listed set,
+    on the sample, and against the AS mode's fixed rule (the fallback); the
+    fastest is installed.  Returns the installed table, or None when no
+    weights are stored."""
+    w = load_weights()
+    if w is None:
+        return None
+    d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(offsets)
+    n = offsets.size - 1
+    rec = collect(scratch, d_buf, d_off, n, cap=1 << 22, stride=stride)
+    learned = policy_from_log(scratch, w, from_log(rec))
+    best = None
+    for table in (learned, as_on_listed(learned), as_rule_table()):
+        scratch.set_policy(table)
+        ms = float("inf")
+        for _ in range(reps + 1):
+            scratch.scan_device(d_buf.ptr, d_off.ptr, n, E.KIND_ENGINE, E.MODE_LEARNED)
+            scratch.collect(n)
+            ms = min(ms, scratch.last_stats()[0])
+        if best is None or ms < best[0]:
+            best = (ms, table)
+    scratch.set_policy(best[1])
+    return best[1]
+
+
 def apply(scratch: E.Scratch, every: int = 0) -> bool:
     """Loads the stored weights, and the policy table derived from them for
     this rule set when one is stored, into the scratch; False when no

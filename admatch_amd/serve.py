@@ -62,6 +62,44 @@ def serve_trace(s: E.Scratch, buf: np.ndarray, offsets: np.ndarray, arrival_s: n
     return lat[:n], E.unpack(out_off, ids[: out_off[-1]]), info
 
 
+def serve_ring(s: E.Scratch, buf: np.ndarray, offsets: np.ndarray, arrival_s: np.ndarray,
+               mode: int = E.MODE_LEARNED):
+    """Device-side serving (ita_serve_ring): one persistent launch whose
+    waves take each request as soon as the host publishes its arrival."""
+    L = E.lib()
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    L.ita_serve_ring.restype = i32
+    L.ita_serve_ring.argtypes = [vp, i32, vp, vp, vp, i32, vp, vp, vp, i64, vp]
+    buf = np.ascontiguousarray(buf, dtype=np.uint8)
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    arrival_s = np.ascontiguousarray(arrival_s, dtype=np.float64)
+    n = offsets.size - 1
+    lat = np.zeros(max(n, 1), dtype=np.float64)
+    out_off = np.zeros(n + 1, dtype=np.int32)
+    stats = np.zeros(6, dtype=np.float64)
+    cap = max(4 * n, 1024)
+    ids = np.zeros(cap, dtype=np.int32)
+    rc = L.ita_serve_ring(s._h, mode, buf.ctypes.data, offsets.ctypes.data, arrival_s.ctypes.data, n,
+                          lat.ctypes.data, out_off.ctypes.data, ids.ctypes.data, cap, stats.ctypes.data)
+    if rc == E.ITA_ERR_CAPACITY:  # the results are kept by the handle
+        ids = np.zeros(int(out_off[-1]), dtype=np.int32)
+        L.ita_last_results(s._h, ids.ctypes.data, ids.size)
+    else:
+        E._err(rc)
+    info = {"span_s": stats[0], "rescanned": int(stats[1]), "waves": int(stats[2]),
+            "method_steps": [int(x) for x in stats[3:6]], "bytes": int(offsets[-1] - offsets[0])}
+    return lat[:n], E.unpack(out_off, ids[: out_off[-1]]), info
+
+
+def summarize_ring(lat: np.ndarray, info: dict, arrival_s: np.ndarray) -> dict:
+    n = lat.size
+    span = max(info["span_s"], 1e-9)
+    return {"p50_us": float(np.percentile(lat, 50) * 1e6), "p99_us": float(np.percentile(lat, 99) * 1e6),
+            "mean_us": float(lat.mean() * 1e6), "offered_req_per_s": float(n / max(arrival_s[-1] - arrival_s[0], 1e-9)),
+            "sustained_req_per_s": n / span, "gbs": info["bytes"] / span / 1e9, "waves": info["waves"],
+            "method_steps": info["method_steps"], "rescanned": info["rescanned"]}
+
+
 def summarize(lat: np.ndarray, info: dict, n: int) -> dict:
     return {"p50_us": float(np.percentile(lat, 50) * 1e6), "p99_us": float(np.percentile(lat, 99) * 1e6),
             "mean_us": float(lat.mean() * 1e6), "req_per_s": n / info["wall_s"],

@@ -50,6 +50,8 @@ def main(argv=None):
     ap.add_argument("--autotune-out", default=autotune.CACHE, help="autotune cache to update")
     ap.add_argument("--extra-medium", type=int, default=4096,
                     help="also log this many strings on the medium rule set (0: large only)")
+    ap.add_argument("--extra-small", type=int, default=2048,
+                    help="also log this many strings on the small rule set (0: none)")
     args = ap.parse_args(argv)
 
     t0 = time.time()
@@ -86,6 +88,16 @@ def main(argv=None):
         dm = (E.DeviceArray(mw.inputs.buf), E.DeviceArray(mw.inputs.offsets))
         rec = np.concatenate([rec, selector.collect(ms, *dm, mw.inputs.n, cap=1 << 20, stride=args.stride // 2)])
         ms.close()
+    if args.extra_small > 0:
+        # and a small one (64 rules, W = 33 words)
+        sw = synth.make_workload("small", n=args.extra_small, seed_offset=1000)
+        spath = os.path.join(d, "small.nfa")
+        sw.ruleset.write(spath)
+        ss = E.Scratch(E.AD_MATCHER, spath).alloc_scratch()
+        autotune.apply(ss)
+        dsm = (E.DeviceArray(sw.inputs.buf), E.DeviceArray(sw.inputs.offsets))
+        rec = np.concatenate([rec, selector.collect(ss, *dsm, sw.inputs.n, cap=1 << 20, stride=max(1, args.stride // 4))])
+        ss.close()
     ds = selector.from_log(rec)
     log(f"training log: {ds.n} steps in {time.time() - t0:.1f} s; best method counts "
         f"{np.bincount(ds.best(), minlength=3).tolist()}")

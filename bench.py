@@ -29,7 +29,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# The serving measurement runs four lanes on streams of their own.  HIP gives
+# The plain-iNFA serving baseline runs four lanes on streams of their own.  HIP gives
 # a process GPU_MAX_HW_QUEUES hardware queues (4 by default) and streams beyond
 # that share them: with the engine's own stream and torch's already holding
 # queues, lanes would pair up and a short batch would wait behind a long one
@@ -62,11 +62,33 @@ def time_mode(s, E, d_buf, d_off, n, kind, mode, steps, warmup):
     return step_ms, kern_ms, res
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(path, inputs, budget_s=12.0, threads=1):
     """The oracle on a bounded prefix of the same batch (strings split
-    across `threads` threads)."""
+    across `threads` threads).  One thread runs pinned to one core of the
+    process's affinity set; more threads run on the whole set."""
     import oracle
 
+    allowed = sorted(os.sched_getaffinity(0))
+    if threads == 1:
+        os.sched_setaffinity(0, {allowed[0]})
+    try:
+        return _cpu_baseline(oracle, path, inputs, budget_s, threads, allowed)
+    finally:
+        os.sched_setaffinity(0, set(allowed))
+
+
+def _cpu_baseline(oracle, path, inputs, budget_s, threads, allowed):
     o = oracle.OracleNFA(path)
     buf, off = inputs.buf, inputs.offsets
     n = off.size - 1
@@ -82,9 +104,84 @@ def cpu_baseline(path, inputs, budget_s=12.0, threads=1):
     o.batch_scan(buf, sub, nthreads=threads)
     dt = time.perf_counter() - t0
     nbytes = int(sub[-1] - sub[0])
-    what = "single-threaded oracle" if threads == 1 else f"oracle on {threads} threads"
+    what = (f"single-threaded oracle pinned to core {allowed[0]}" if threads == 1
+            else f"oracle on {threads} threads over the {len(allowed)} cores of the affinity set")
     return {"value": round(nbytes / dt / 1e9, 6), "unit": "GB/s", "cores": threads, "kind": "port",
-            "sample": f"first {want} of {n} strings ({nbytes} B), {what}, {dt:.1f} s"}
+            "sample": f"first {want} of {n} strings ({nbytes} B), {what}, {dt:.1f} s",
+            "cpu_model": cpu_model(), "cores_available": len(allowed), "os_cpu_count": os.cpu_count()}
+
+
+def density_histogram(s, E, d_buf, d_off, n, S):
+    """Active states per scan step as a share of all states, on a sample of
+    the steps (one in 16, timed by the engine's logging mode): the share of
+    steps per density bucket, counting the always-active states."""
+    from admatch_amd import selector
+    rec = selector.collect(s, d_buf, d_off, n, cap=1 << 22, stride=16)
+    act = np.rint(np.exp2(rec[:, 0].astype(np.float64)) - 1)  # total active per logged step
+    frac = act / S
+    edges = [0, 0.001, 0.003, 0.01, 0.03, 0.10, 1.01]
+    names = ["<0.1%", "0.1-0.3%", "0.3-1%", "1-3%", "3-10%", ">=10%"]
+    h, _ = np.histogram(frac, bins=edges)
+    return {"sampled_steps": int(act.size), "share_of_steps": dict(zip(names, (h / max(1, act.size)).round(4).tolist())),
+            "max_active": int(act.max()) if act.size else 0, "max_share": round(float(frac.max()), 4) if act.size else 0,
+            "p50_active": float(np.percentile(act, 50)), "p99_active": float(np.percentile(act, 99))}
+
+
+def pcie_inclusive(s, w, reps=2):
+    """BatchedScan from host strings (pinned staging, host-to-device copy,
+    scan, result copy and rule mapping), the reference's per-call path."""
+    best = float("inf")
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        s.batched_scan_packed(w.inputs.buf, w.inputs.offsets)
+        best = min(best, time.perf_counter() - t0)
+    nbytes = int(w.inputs.offsets[-1])
+    return {"gbs": round(nbytes / best / 1e9, 4), "ms": round(best * 1e3, 2),
+            "what": "ita_batched_scan from host memory, H2D and D2H included (best of %d)" % reps}
+
+
+def other_configs(E, synth, steps):
+    """Input GB/s of the small (64 rules, 1,024 strings) and medium (1,000
+    rules, 16,384 strings) configurations and of the sparse large rule set,
+    inputs resident: plain iNFA (best of 256/512 threads), AS and learned."""
+    from admatch_amd import autotune, selector
+    out = {}
+    for name in ("small", "medium", "large_sparse"):
+        w = synth.make_workload(name)
+        path = os.path.join(tempfile.mkdtemp(), f"{name}.nfa")
+        w.ruleset.write(path)
+        s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+        autotune.apply(s)
+        if selector.load_policy(s.info()["hash"]) is None:
+            # no table stored for this rule set: compile the network into one
+            # on a sample of its traffic drawn apart from the measured strings
+            cal = synth.make_workload(name, n=1024, seed_offset=1)
+            selector.calibrate(s, cal.inputs.buf, cal.inputs.offsets)
+        else:
+            selector.apply(s)
+        n, nbytes = w.inputs.n, int(w.inputs.offsets[-1])
+        d_buf, d_off = E.DeviceArray(w.inputs.buf), E.DeviceArray(w.inputs.offsets)
+        r = {"workload": f"{w.ruleset.state_count} states, {n} strings, {nbytes} B", "gbs": {}}
+        ref = None
+        plain_best = None
+        wpb, cap, _, store = s.variant()
+        for bs in (256, 512):
+            s.set_variant(wpb, cap, bs, store)
+            _, kern_ms, res = time_mode(s, E, d_buf, d_off, n, E.KIND_PLAIN, E.MODE_INFA, max(2, steps // 2), 1)
+            ref = E.unpack(*res)
+            g = nbytes / (float(np.median(kern_ms)) * 1e-3) / 1e9
+            plain_best = g if plain_best is None else max(plain_best, g)
+        r["gbs"]["plain_infa"] = round(plain_best, 4)
+        for mname, mode in (("as", E.MODE_AS), ("learned", E.MODE_LEARNED)):
+            _, kern_ms, res = time_mode(s, E, d_buf, d_off, n, E.KIND_ENGINE, mode, steps, 1)
+            if E.unpack(*res) != ref:
+                raise SystemExit(f"{name}: {mname} disagrees with plain iNFA")
+            r["gbs"][mname] = round(nbytes / (float(np.median(kern_ms)) * 1e-3) / 1e9, 4)
+            r.setdefault("method_steps", {})[mname] = s.last_stats()[1]
+        s.close()
+        out[name] = r
+        log(f"{name}: {r}")
+    return out
 
 
 def readme_workload(E, weights, steps):
@@ -108,7 +205,11 @@ def readme_workload(E, weights, steps):
     # ita_batched_scan follows the handle's flags: the learned selector
     s = E.Scratch(E.AD_MATCHER | E.LEARNED_SELECTOR, path).alloc_scratch()
     if weights is not None:
-        s.set_selector(weights)
+        # the network compiled into a table on the fixture's other requests
+        from admatch_amd import selector
+        cb, co = E.pack_strings([z["buf"][z["offsets"][i]:z["offsets"][i + 1]].tobytes()
+                                 for i in range(1, z["offsets"].size - 1)])
+        selector.calibrate(s, cb, co)
     d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
     out = {"workload": f"CRS group 6 ({s.info()['states']} states) x data/bing.txt request ({len(bing)} B) x {n}",
            "published_us_per_request": {"iNFA": 7258, "AdMatch": 62, "hardware": "not stated (NVIDIA, README.md:19-20)"},
@@ -151,7 +252,9 @@ def frontend_workload(E, synth, weights, steps):
     s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
     info = s.info()
     if weights is not None:
-        s.set_selector(weights)
+        from admatch_amd import selector
+        cal = synth.make_workload("medium", n=1024, seed_offset=1)
+        selector.calibrate(s, cal.inputs.buf, cal.inputs.offsets)
     d_buf, d_off = E.DeviceArray(w.inputs.buf), E.DeviceArray(w.inputs.offsets)
     out = {"workload": f"{len(pats)} patterns ({info['states']} states, {info['wb_transitions']} word-boundary "
                        f"transitions) x {n} strings ({nbytes} B)", "compile_s": round(compile_s, 3), "gbs": {}}
@@ -203,27 +306,50 @@ def streaming_workload(E, path, weights, w, n=16384, parts=4, tune=None):
 
 
 def serving(s, E, synth, args):
-    """Continuous-batching request loop on the large rule set: Poisson
-    arrivals, lognormal lengths (median 800 B, max 16 KB)."""
+    """Serving on the large rule set: Poisson arrivals, lognormal lengths
+    (median 800 B, max 16 KB), args.serve_seconds of arrivals per rate.  The
+    learned engine admits requests on the device (ita_serve_ring: one
+    persistent launch, a wave takes the next published request as it
+    finishes one); plain iNFA runs the same trace through the host-admitted
+    lane loop (ita_serve_trace).  The two must return the same rule ids."""
     from admatch_amd import serve
 
-    w = synth.make_workload("serve", n=args.serve_requests)
-    out = {"workload": f"{w.inputs.n} requests, {int(w.inputs.offsets[-1])} B, lognormal median 800 B",
-           "lanes": 4, "rates": {}}
-    for r in args.serve_rates.split(","):
-        rate = float(r)
-        arr = serve.poisson_arrivals(w.inputs.n, rate, seed=int(rate))
-        lat, res, info = serve.serve_trace(s, w.inputs.buf, w.inputs.offsets, arr, mode=E.MODE_LEARNED, lanes=4)
-        k = min(2000, w.inputs.n)
-        lat2, res2, info2 = serve.serve_trace(s, w.inputs.buf, w.inputs.offsets[: k + 1], arr[:k], kind=E.KIND_PLAIN,
-                                              mode=E.MODE_INFA, lanes=4)
-        if res2 != res[:k]:
-            raise SystemExit("serving: plain iNFA and the learned engine disagree")
-        out["rates"][r] = {"rate": rate, "learned": serve.summarize(lat, info, w.inputs.n),
-                           "plain_infa": serve.summarize(lat2, info2, k)}
-        log(f"serving {rate:.0f} req/s: learned p50 {out['rates'][r]['learned']['p50_us']:.0f} us "
-            f"p99 {out['rates'][r]['learned']['p99_us']:.0f} us; plain p50 {out['rates'][r]['plain_infa']['p50_us']:.0f} "
-            f"p99 {out['rates'][r]['plain_infa']['p99_us']:.0f} us")
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    n_max = int(max(rates) * args.serve_seconds)
+    t0 = time.time()
+    w = synth.make_workload("serve", n=n_max)
+    log(f"serving trace: {n_max} requests generated in {time.time() - t0:.1f} s")
+    buf, off = w.inputs.buf, w.inputs.offsets
+    # a request's own scan time: the median-length request scanned alone
+    lens = np.diff(off)
+    med = int(np.argsort(lens)[lens.size // 2])
+    b1, o1 = E.pack_strings([w.inputs.string(med)])
+    d1, do1 = E.DeviceArray(b1), E.DeviceArray(o1)
+    own = []
+    for _ in range(21):
+        s.scan_device(d1.ptr, do1.ptr, 1, E.KIND_ENGINE, E.MODE_LEARNED)
+        s.collect(1)
+        own.append(s.last_stats()[0])
+    own_us = float(np.median(own[1:])) * 1e3
+    out = {"workload": f"{n_max} requests drawn, lognormal median 800 B, max 16 KB, "
+                       f"{args.serve_seconds:g} s of arrivals per rate",
+           "median_request": {"bytes": int(lens[med]), "own_scan_us": round(own_us, 1)}, "rates": {}}
+    for rate in rates:
+        n = int(rate * args.serve_seconds)
+        arr = serve.poisson_arrivals(n, rate, seed=int(rate))
+        sub = off[: n + 1]
+        lat, res, info = serve.serve_ring(s, buf, sub, arr, mode=E.MODE_LEARNED)
+        ring = serve.summarize_ring(lat, info, arr)
+        ring["p50_over_own_scan"] = round(ring["p50_us"] / own_us, 3)
+        lat2, res2, info2 = serve.serve_trace(s, buf, sub, arr, kind=E.KIND_PLAIN, mode=E.MODE_INFA, lanes=4)
+        if res2 != res:
+            raise SystemExit(f"serving at {rate:.0f} req/s: plain iNFA and the learned engine disagree")
+        key = str(int(rate))
+        out["rates"][key] = {"rate": rate, "requests": n, "learned_device_admission": ring,
+                             "plain_infa_lanes": serve.summarize(lat2, info2, n)}
+        log(f"serving {rate:.0f} req/s ({n} requests): learned p50 {ring['p50_us']:.0f} us p99 {ring['p99_us']:.0f} us "
+            f"sustained {ring['sustained_req_per_s']:.0f} req/s; plain p50 "
+            f"{out['rates'][key]['plain_infa_lanes']['p50_us']:.0f} us")
     return out
 
 
@@ -236,8 +362,8 @@ def main():
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--modes", default="plain,admatch,heuristic,as,learned")
-    ap.add_argument("--serve-requests", type=int, default=20000,
-                    help="requests per arrival rate for the serving measurement (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=1.0,
+                    help="seconds of arrivals per rate for the serving measurement (0: skip)")
     ap.add_argument("--serve-rates", default="50000,200000,800000")
     ap.add_argument("--no-streaming", dest="streaming", action="store_false",
                     help="skip the streaming measurement (16,384 strings as 4-chunk streams)")
@@ -245,6 +371,8 @@ def main():
                     help="skip the regex front-end workload (1,000 compiled patterns x 16,384 strings)")
     ap.add_argument("--no-readme", dest="readme", action="store_false",
                     help="skip the reference README workload (CRS group 6 x bing.txt x 1024)")
+    ap.add_argument("--no-extras", dest="extras", action="store_false",
+                    help="skip the density histogram, the host-strings rate and the other configurations")
     ap.add_argument("--variant", default=None,
                     help="wpb,cap,store[,plain_bs]: launch variant (default: the autotuned one)")
     ap.add_argument("--tune", default=None, metavar="PATH",
@@ -269,7 +397,9 @@ def main():
     path = os.path.join(tmp, f"{args.config}.nfa")
     w.ruleset.write(path)
 
-    s = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    # BatchedScan through this handle (the host-strings rate) follows its
+    # flags: the learned selector
+    s = E.Scratch(E.AD_MATCHER | E.LEARNED_SELECTOR, path).alloc_scratch()
     info = s.info()
     tuned = None
     tune_path = os.path.join(ROOT, "admatch_amd", "data", "autotune.json")
@@ -366,16 +496,22 @@ def main():
     if args.streaming and args.config == "large":
         line["streaming_workload"] = streaming_workload(E, path, np.load(SELECTOR_PATH) if learned_ok else None, w,
                                                         tune=tuned)
-    if args.serve_requests > 0 and args.config == "large":
+    if args.serve_seconds > 0 and args.config == "large":
         line["serving"] = serving(s, E, synth, args)
-        mid = line["serving"]["rates"][args.serve_rates.split(",")[len(args.serve_rates.split(",")) // 2]]
-        line["p50_us_per_request"] = round(mid["learned"]["p50_us"], 2)
-        line["p99_us_per_request"] = round(mid["learned"]["p99_us"], 2)
-        line["latency_source"] = f"serving loop at {mid['rate']:.0f} req/s (per-request completion - arrival)"
+        mid = line["serving"]["rates"][str(int(float(args.serve_rates.split(",")[len(args.serve_rates.split(",")) // 2])))]
+        line["p50_us_per_request"] = round(mid["learned_device_admission"]["p50_us"], 2)
+        line["p99_us_per_request"] = round(mid["learned_device_admission"]["p99_us"], 2)
+        line["latency_source"] = (f"device-admitted serving at {mid['rate']:.0f} req/s "
+                                  "(per-request completion seen by the host - arrival)")
+    if args.extras and args.config == "large":
+        log("density histogram, host-strings rate, other configurations")
+        line["active_density"] = density_histogram(s, E, d_buf, d_off, n, w.ruleset.state_count)
+        line["pcie_inclusive"] = pcie_inclusive(s, w)
+        line["other_configs"] = other_configs(E, synth, max(3, args.steps // 2))
     if not args.no_cpu:
         log("cpu baseline (oracle, 1 thread)")
         line["cpu_baseline"] = cpu_baseline(path, w.inputs)
-        threads = max(1, min(16, os.cpu_count() or 1))  # the box's CPU share for one GPU
+        threads = len(os.sched_getaffinity(0))  # every core the process may run on
         log(f"cpu baseline (oracle, {threads} threads)")
         line["cpu_baseline_all_cores"] = cpu_baseline(path, w.inputs, budget_s=6.0, threads=threads)
     s.close()

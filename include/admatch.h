@@ -245,6 +245,23 @@ int ita_serve_trace(ita_handle* h, int32_t kind, int32_t mode, int32_t lanes, in
                     const int64_t* offsets, const double* arrival_s, int32_t n, double* latency_s,
                     int32_t* out_offsets, int32_t* out_ids, int64_t cap, double* stats6);
 
+/* Device-side serving (no counterpart in the reference): replays n requests
+ * (buf[offsets[i] .. offsets[i+1]), arriving arrival_s[i] seconds after the
+ * start, non-decreasing) through ONE persistent engine launch.  The host
+ * publishes each request when it arrives; a wave that finishes a request
+ * takes the next published one, copies its bytes from host memory and scans
+ * it, and writes the accepted states and a done flag to host memory, which
+ * the host polls.  Every wait in the kernel is bounded (last arrival + 10 s).
+ * The launch holds the whole GPU until the last request completes; run no
+ * other work on the device meanwhile.  mode: ITA_MODE_* (engine).  Per
+ * request: latency_s[i] (completion seen by the host - arrival) and rule ids
+ * in out_ids[out_offsets[i] .. out_offsets[i+1]).  stats6: [0] seconds from
+ * the first arrival to the last completion, [1] requests scanned again after
+ * the accepted-state buffer overflowed, [2] waves in the launch, [3..5] steps
+ * per method (iNFA, TKO, AS). */
+int ita_serve_ring(ita_handle* h, int32_t mode, const uint8_t* buf, const int64_t* offsets, const double* arrival_s,
+                   int32_t n, double* latency_s, int32_t* out_offsets, int32_t* out_ids, int64_t cap, double* stats6);
+
 /* Message of this thread's last failed call. */
 const char* ita_last_error(void);
 
