@@ -277,6 +277,19 @@ int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3) {
     return ITA_OK;
 }
 
+int ita_last_times(ita_handle* h, float* out6) {
+    if (int rc = check(h)) return rc;
+    if (!out6) return fail(ITA_ERR_ARG, "null output");
+    const auto& st = h->eng.stats();
+    out6[0] = st.kernel_ms;
+    out6[1] = st.engine_ms;
+    out6[2] = st.alloc_ms;
+    out6[3] = st.h2d_ms;
+    out6[4] = st.d2h_ms;
+    out6[5] = st.map_ms;
+    return ITA_OK;
+}
+
 int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs, int32_t store) {
     if (int rc = check(h)) return rc;
     if ((wpb != 1 && wpb != 2 && wpb != 4) || (cap != 128 && cap != 256 && cap != 512 && cap != 1024) ||

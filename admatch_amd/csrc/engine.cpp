@@ -30,6 +30,16 @@ static void hfree(T*& p) {
     p = nullptr;
 }
 
+// Elapsed time between two recorded events into *out (left unchanged when HIP
+// has none); a failed query leaves no error behind for the next launch check.
+static void elapsed_ms(hipEvent_t a, hipEvent_t b, float* out) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) == hipSuccess)
+        *out = ms;
+    else
+        (void)hipGetLastError();
+}
+
 // ------------------------------------------------------------------ Context
 Context::~Context() {
     release();
@@ -38,7 +48,7 @@ Context::~Context() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
     if (done) (void)hipEventDestroy(done);
-    for (hipEvent_t e : {ev_h2d0, ev_h2d1, ev_d2h0, ev_d2h1})
+    for (hipEvent_t e : {ev_h2d0, ev_h2d1, ev_d2h0, ev_d2h1, ev_k0, ev_k1})
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -54,6 +64,8 @@ int Context::init(std::string& err) {
     ADM_HIP(hipEventCreate(&ev_h2d1));
     ADM_HIP(hipEventCreate(&ev_d2h0));
     ADM_HIP(hipEventCreate(&ev_d2h1));
+    ADM_HIP(hipEventCreate(&ev_k0));
+    ADM_HIP(hipEventCreate(&ev_k1));
     ADM_HIP(hipMalloc(&d_ctr, kCtrWords * sizeof(int32_t)));
     ADM_HIP(hipHostMalloc(&h_ctr, kCtrWords * sizeof(int32_t)));
     return ADM_OK;
@@ -399,10 +411,14 @@ ScanArgs Engine::make_args(Context& c, const uint8_t* d_text, const int64_t* d_o
 }
 
 int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind,
-                   int32_t mode, std::string& err) {
+                   int32_t mode, std::string& err, LaunchPhase phase) {
     ScanArgs a = make_args(c, d_text, d_off, n, mode);
     hipStream_t st = c.stream;
-    ADM_HIP(launch_fill(c.d_ctr, 0, kCtrWords, st));
+    const bool prefix = phase != kPhaseEngine, engine = phase != kPhasePrefix;
+    if (prefix) {
+        c.engine_timed = false;
+        ADM_HIP(launch_fill(c.d_ctr, 0, kCtrWords, st));
+    }
     if (n == 0) return ADM_OK;
     if (kind == KIND_PLAIN) {
         const size_t lds = plain_lds_bytes();
@@ -411,7 +427,7 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
             return ADM_ERR_TOO_LARGE;
         }
         a.order = nullptr;
-        ADM_HIP(launch_plain(var_.plain_bs, n, lds, st, dev_, a));
+        if (prefix) ADM_HIP(launch_plain(var_.plain_bs, n, lds, st, dev_, a));
         return ADM_OK;
     }
     if (mode == MODE_LOG && (!d_log_ || log_cap_ <= 0)) {
@@ -428,11 +444,15 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
         }
         a.dense = c.d_dense;
     }
-    ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
+    if (prefix) ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
+    if (!engine) return ADM_OK;
     const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
     // lean hashed kernels for rule sets without boundaries and with two-byte rows
     const int32_t kmode = (hs && g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
+    ADM_HIP(hipEventRecord(c.ev_k0, st));
     ADM_HIP(launch_engine(var_.wpb, var_.cap, kmode, hs, grid, lds, st, dev_, a));
+    ADM_HIP(hipEventRecord(c.ev_k1, st));
+    c.engine_timed = true;
     return ADM_OK;
 }
 
@@ -524,7 +544,7 @@ int Engine::scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off,
             hipGraph_t gr = nullptr;
             ADM_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             std::string lerr;
-            int rc = launch(c, d_text, d_off, n, kind, mode, lerr);
+            int rc = launch(c, d_text, d_off, n, kind, mode, lerr, kPhasePrefix);
             hipError_t ec = hipStreamEndCapture(st, &gr);
             if (rc) {
                 if (gr) (void)hipGraphDestroy(gr);
@@ -545,6 +565,8 @@ int Engine::scan_device(Context& c, const uint8_t* d_text, const int64_t* d_off,
             it = c.graphs.emplace(key, ex).first;
         }
         ADM_HIP(hipGraphLaunch(it->second, st));
+        c.engine_timed = false;
+        if (int rc = launch(c, d_text, d_off, n, kind, mode, err, kPhaseEngine)) return rc;
     } else {
         if (int rc = launch(c, d_text, d_off, n, kind, mode, err)) return rc;
     }
@@ -603,8 +625,9 @@ int Engine::begin_collect(Context& c, int32_t n, std::string& err) {
 int Engine::finish_collect(Context& c, int32_t n, std::vector<int32_t>& out_off, std::vector<int32_t>& out_ids,
                            std::string& err) {
     ADM_HIP(hipEventSynchronize(c.done));
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, c.ev0, c.ev1) == hipSuccess) c.stats.kernel_ms = ms;
+    elapsed_ms(c.ev0, c.ev1, &c.stats.kernel_ms);
+    c.stats.engine_ms = 0.f;
+    if (c.engine_timed) elapsed_ms(c.ev_k0, c.ev_k1, &c.stats.engine_ms);
     unsigned long long top = 0;
     std::memcpy(&top, c.h_ctr + kCtrPoolTop, 8);
     std::memcpy(c.stats.method_steps, c.h_ctr + kCtrMethods, sizeof(c.stats.method_steps));
@@ -621,7 +644,7 @@ int Engine::finish_collect(Context& c, int32_t n, std::vector<int32_t>& out_off,
                                c.stream));
     ADM_HIP(hipEventRecord(c.ev_d2h1, c.stream));
     ADM_HIP(hipStreamSynchronize(c.stream));
-    if (hipEventElapsedTime(&ms, c.ev_d2h0, c.ev_d2h1) == hipSuccess) c.stats.d2h_ms = ms;
+    elapsed_ms(c.ev_d2h0, c.ev_d2h1, &c.stats.d2h_ms);
     const auto tm = std::chrono::steady_clock::now();
     if (c.emit_all) {
         // every final state, ascending (the reference's get_active_states,
@@ -668,7 +691,6 @@ int Engine::scan_host(const uint8_t* const* text, const int32_t* len, int32_t n,
     }
     ctx_.stats.alloc_ms = 0.f;
     if (int rc = stage(ctx_, text, len, n, err)) return rc;
-    float h2d = 0.f;
     // a pool overflow grows the pool to fit, so the second scan succeeds
     ctx_.emit_all = final_states;
     int rc = ADM_ERR_POOL;
@@ -678,7 +700,7 @@ int Engine::scan_host(const uint8_t* const* text, const int32_t* len, int32_t n,
         rc = collect(ctx_, n, out_off, out_ids, err);
     }
     ctx_.emit_all = false;
-    if (rc == ADM_OK && hipEventElapsedTime(&h2d, ctx_.ev_h2d0, ctx_.ev_h2d1) == hipSuccess) ctx_.stats.h2d_ms = h2d;
+    if (rc == ADM_OK) elapsed_ms(ctx_.ev_h2d0, ctx_.ev_h2d1, &ctx_.stats.h2d_ms);
     return rc;
 }
 

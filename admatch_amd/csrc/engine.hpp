@@ -45,6 +45,7 @@ struct Variant {
 
 struct ScanStats {
     float kernel_ms = 0.f;  // last scan: hipEvent time of the launch sequence
+    float engine_ms = 0.f;  // last engine scan: hipEvent time of the engine kernel alone
     // phases of the last host-strings scan (the reference's PROFILER_MODE,
     // src/kernel_common.cu:57-95): device buffer growth (host clock; buffers
     // are reused, so usually 0), host-to-device copies and result copies
@@ -70,6 +71,10 @@ struct Context {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, done = nullptr;
     hipEvent_t ev_h2d0 = nullptr, ev_h2d1 = nullptr, ev_d2h0 = nullptr, ev_d2h1 = nullptr;
+    // events around the engine kernel; it is always launched outside a captured
+    // graph, since HIP gives no elapsed time for events recorded in one
+    hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr;
+    bool engine_timed = false;  // the last launch sequence recorded ev_k0/ev_k1
     // emit every final state, not only the accepting ones; the last scan's
     // final states of string i are fin_states[fin_off[i] .. fin_off[i+1])
     bool emit_all = false;
@@ -102,6 +107,7 @@ struct Context {
     uint8_t* d_stream_out_flags = nullptr;
     int32_t stream_n_cap = 0;
     int ensure_stream(int32_t n, int32_t W, std::string& err);
+    // captured launch prefixes (counter clear, length binning, the plain kernel)
     std::map<std::tuple<const void*, const void*, int32_t, int32_t, int32_t>, hipGraphExec_t> graphs;
     ScanStats stats;
 
@@ -223,8 +229,11 @@ class Engine {
 
    private:
     int engine_grid(int32_t mode, int32_t n, bool hashed_storage, int* grid, std::string& err);
+    // phases of a launch sequence: the prefix is graph-capturable, the engine
+    // kernel runs between ev_k0 and ev_k1
+    enum LaunchPhase { kPhaseAll = 0, kPhasePrefix = 1, kPhaseEngine = 2 };
     int launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, KernelKind kind, int32_t mode,
-               std::string& err);
+               std::string& err, LaunchPhase phase = kPhaseAll);
     ScanArgs make_args(Context& c, const uint8_t* d_text, const int64_t* d_off, int32_t n, int32_t mode);
     void drop_all_graphs();
 

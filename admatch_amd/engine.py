@@ -44,6 +44,7 @@ EXPORTED = [
     "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
     "ita_selector_forward", "ita_serve_trace", "ita_stream_create", "ita_stream_reset", "ita_stream_scan",
     "ita_stream_destroy", "ita_last_results", "ita_scan_device_noalloc", "ita_set_policy", "ita_serve_ring",
+    "ita_last_times",
 ]
 
 _LIB = None
@@ -73,6 +74,7 @@ def lib():
             "ita_collect": (i32, [vp, i32, vp, vp, i64]),
             "ita_scan_device_noalloc": (i32, [vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i64, vp]),
             "ita_last_stats": (i32, [vp, vp, vp]),
+            "ita_last_times": (i32, [vp, vp]),
             "ita_set_variant": (i32, [vp, i32, i32, i32, i32]),
             "ita_get_variant": (i32, [vp, vp, vp, vp, vp]),
             "ita_set_selector": (i32, [vp, vp, i32, i32]),
@@ -216,6 +218,13 @@ class Scratch:
         steps = np.zeros(3, dtype=np.uint64)
         _err(lib().ita_last_stats(self._h, ctypes.byref(ms), _p(steps)))
         return ms.value, steps.tolist()
+
+    def last_times(self) -> dict:
+        """ms of the last scan: launch sequence, engine kernel alone, and the
+        host-strings path's phases."""
+        t = np.zeros(6, dtype=np.float32)
+        _err(lib().ita_last_times(self._h, _p(t)))
+        return dict(zip(("sequence", "engine_kernel", "alloc", "h2d", "d2h", "map"), map(float, t)))
 
     # -- configuration ---------------------------------------------------
     def set_variant(self, wpb: int = 2, cap: int = 512, plain_bs: int = 64, store: int = 0):

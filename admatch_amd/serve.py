@@ -1,13 +1,15 @@
-"""Serving loop driver: Poisson request traces replayed through the native
-continuous-batching loop (ita_serve_trace, admatch_amd/csrc/serve.cpp).
+"""Serving drivers: Poisson request traces replayed through the native loops
+(admatch_amd/csrc/serve.cpp).
 
-    python -m admatch_amd.serve --rates 50000,200000,800000 --requests 40000
+    python -m admatch_amd.serve --rates 50000,200000,800000 --seconds 1 [--plain]
 
 Requests are strings of the serving workload (the large 3,000-rule set,
 lognormal lengths with median 800 B, max 16 KB).  For each arrival rate the
-learned-selector engine and the plain iNFA path are run on the same trace;
-reported per rate: p50/p99 latency, achieved request and byte throughput,
-and the batching the loop formed.
+learned-selector engine serves the trace with device-side admission
+(ita_serve_ring: one persistent launch, a wave takes the next published
+request as it finishes one); with --plain, the plain iNFA kernel serves the
+same trace through the host-admitted lane loop (ita_serve_trace).  Reported
+per rate: p50/p99 latency and the sustained request and byte rates.
 """
 from __future__ import annotations
 
@@ -119,34 +121,32 @@ def hw_queues_for(lanes: int) -> None:
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--rates", default="50000,200000,800000")
-    ap.add_argument("--requests", type=int, default=40000)
-    ap.add_argument("--lanes", type=int, default=4)
-    ap.add_argument("--plain-requests", type=int, default=4000)
+    ap.add_argument("--seconds", type=float, default=1.0, help="seconds of arrivals per rate")
+    ap.add_argument("--plain", action="store_true", help="also run plain iNFA through the lane loop")
     args = ap.parse_args(argv)
-    hw_queues_for(args.lanes)
-    w = synth.make_workload("serve", n=args.requests)
+    hw_queues_for(4)
+    rates = [float(r) for r in args.rates.split(",")]
+    w = synth.make_workload("serve", n=int(max(rates) * args.seconds))
     d = tempfile.mkdtemp()
     path = os.path.join(d, "serve.nfa")
     w.ruleset.write(path)
     s = E.Scratch(E.AD_MATCHER | E.LEARNED_SELECTOR, path).alloc_scratch()
     autotune.apply(s)
-    wts = selector.load_weights()
-    if wts is not None:
-        selector.apply(s)
+    learned = selector.apply(s)
     report = {"workload": f"{w.ruleset.state_count} states, {w.inputs.n} requests, "
-                          f"{int(w.inputs.offsets[-1])} B", "learned_selector": wts is not None, "rates": {}}
-    for rate in (float(r) for r in args.rates.split(",")):
-        arr = poisson_arrivals(w.inputs.n, rate, seed=int(rate))
-        lat, res, info = serve_trace(s, w.inputs.buf, w.inputs.offsets, arr, mode=E.MODE_LEARNED,
-                                     lanes=args.lanes)
-        eng = summarize(lat, info, w.inputs.n)
-        # plain iNFA on a prefix of the same trace
-        k = min(args.plain_requests, w.inputs.n)
-        lat2, res2, info2 = serve_trace(s, w.inputs.buf, w.inputs.offsets[: k + 1], arr[:k], kind=E.KIND_PLAIN,
-                                        mode=E.MODE_INFA, lanes=args.lanes)
-        assert res2 == res[:k], "plain iNFA and the engine disagree"
-        report["rates"][str(int(rate))] = {"engine": eng, "plain_infa": summarize(lat2, info2, k)}
-        print(f"[serve] rate {rate:.0f}/s engine {eng}", file=sys.stderr, flush=True)
+                          f"{int(w.inputs.offsets[-1])} B", "learned_selector": learned, "rates": {}}
+    for rate in rates:
+        n = int(rate * args.seconds)
+        arr = poisson_arrivals(n, rate, seed=int(rate))
+        sub = w.inputs.offsets[: n + 1]
+        lat, res, info = serve_ring(s, w.inputs.buf, sub, arr, mode=E.MODE_LEARNED)
+        r = {"engine_device_admission": summarize_ring(lat, info, arr)}
+        if args.plain:
+            lat2, res2, info2 = serve_trace(s, w.inputs.buf, sub, arr, kind=E.KIND_PLAIN, mode=E.MODE_INFA, lanes=4)
+            assert res2 == res, "plain iNFA and the engine disagree"
+            r["plain_infa_lanes"] = summarize(lat2, info2, n)
+        report["rates"][str(int(rate))] = r
+        print(f"[serve] rate {rate:.0f}/s {r}", file=sys.stderr, flush=True)
     print(json.dumps(report))
     s.close()
 

@@ -153,8 +153,15 @@ int64_t ita_last_results(ita_handle* h, int32_t* out_ids, int64_t cap);
  * (iNFA, TKO, AS) of the last engine scan. */
 int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3);
 
+/* Times (ms) of the handle's last scan: [0] the launch sequence (as
+ * ita_last_stats), [1] the engine kernel alone (hipEvents around it, also
+ * inside a replayed graph; 0 for the plain kernel), and for the last
+ * host-strings scan [2] device buffer growth, [3] host-to-device copies,
+ * [4] result copies, [5] the host's accept-map lookup. */
+int ita_last_times(ita_handle* h, float* out6);
+
 /* Engine launch variant: waves (strings) per workgroup in {1,2,4}, active-list
- * capacity in {128,256,512,1024}; plain_bs in {64,128,256} for the plain kernel;
+ * capacity in {128,256,512,1024}; plain_bs in {64,...,1024} for the plain kernel;
  * store selects the AS and learned modes' state storage: 0 = two S-bit
  * bitmaps per string in LDS, 1 = active lists with LDS hash tables (LDS per
  * string independent of S) and the S-bit bitmaps in device memory, used by
