@@ -304,6 +304,13 @@ int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs, i
     return ITA_OK;
 }
 
+int ita_set_tko_threshold(ita_handle* h, int32_t threshold) {
+    if (int rc = check(h)) return rc;
+    std::string err;
+    if (int rc = h->eng.set_tko_threshold(threshold, err)) return fail(rc, err);
+    return ITA_OK;
+}
+
 int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs, int32_t* store) {
     if (int rc = check(h)) return rc;
     const auto& v = h->eng.variant();

@@ -209,11 +209,31 @@ int Engine::load(const char* path, std::string& err) {
     }
     Graph g;
     if (load_nfa(path, g, err) != 0) return ADM_ERR_LOAD;
-    build_tko(g);
+    build_tko(g, resolve_tko(g));
     build_as(g);
     build_wb_rows(g);
     g_ = std::move(g);
     loaded_ = true;
+    return ADM_OK;
+}
+
+int32_t Engine::resolve_tko(const Graph& g) const {
+    if (tko_thr_ >= 0) return tko_thr_;
+    if (tko_thr_ == kTkoReference) return g.S / 16;
+    return g.W / 2;
+}
+
+int Engine::set_tko_threshold(int32_t thr, std::string& err) {
+    if (thr < kTkoAuto) {
+        err = "TKO threshold must be >= 0, -1 (the reference's) or -2 (auto)";
+        return ADM_ERR_ARG;
+    }
+    if (dev_ready_) {
+        err = "set the TKO threshold before the tables go to the device";
+        return ADM_ERR_STATE;
+    }
+    tko_thr_ = thr;
+    if (loaded_) build_tko(g_, resolve_tko(g_));
     return ADM_OK;
 }
 

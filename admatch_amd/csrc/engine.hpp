@@ -170,6 +170,14 @@ class Engine {
     void forget_context(Context* c);
 
     void set_variant(const Variant& v);
+    // Grouping threshold of the TKO shift groups: an offset group becomes a
+    // shift row when it holds more than this many transitions.  kTkoReference
+    // is the reference's state_count/16; kTkoAuto (the default) is W/2, where a
+    // row of W words costs about what W/2 scattered edges cost.  Results do
+    // not depend on it.  Set before the tables go to the device.
+    static constexpr int32_t kTkoReference = -1, kTkoAuto = -2;
+    int set_tko_threshold(int32_t thr, std::string& err);
+    int32_t tko_threshold_setting() const { return tko_thr_; }
     const Variant& variant() const { return var_; }
     int set_mlp(const float* w, int32_t n, int32_t every, std::string& err);
     // streaming: chunk i continues stream ids[i] (distinct within a call);
@@ -243,6 +251,8 @@ class Engine {
     std::vector<void*> dev_allocs_;
     size_t table_bytes_ = 0;
     int64_t tko_rows_ = 0;
+    int32_t tko_thr_ = kTkoAuto;
+    int32_t resolve_tko(const Graph& g) const;
     int cu_count_ = 0;
     size_t max_lds_ = 0;
     Variant var_;

@@ -267,6 +267,25 @@ __device__ int scatter(const uint2* edges, int cnt, const uint64_t* cur, uint64_
     return n;
 }
 
+// scatter with the first round's U edges per lane already loaded
+template <int U>
+__device__ __forceinline__ void scatter_pre(const uint2 (&e0)[U], const uint2* edges, int cnt, const uint64_t* cur,
+                                            uint64_t* fut) {
+    for (int t0 = 0; t0 < cnt; t0 += 64 * U) {
+        uint2 e[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) e[u] = t0 == 0 ? e0[u] : edges[min(t0 + u * 64 + lane_id(), cnt - 1)];
+        uint64_t w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) w[u] = cur[e[u].x >> 6];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const bool hit = t0 + u * 64 + lane_id() < cnt && ((w[u] >> (e[u].x & 63)) & 1ull);
+            if (hit) set_bit_atomic(fut, e[u].y);
+        }
+    }
+}
+
 // TKO shift groups of symbol c: fut[j] = OR over groups i of shift(lim_i &
 // cur, d_i)[j], fut zero on entry.  Lane l owns word j = j0 + 64u + l of U
 // consecutive 64-word chunks and accumulates every group in registers, so
@@ -692,9 +711,25 @@ __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, 
     int n = 0;
     const int W = g.W;
     if (m == M_TKO) {
-        tko_groups<8>(g, c, cur, fut);
+        // the leftover edges' first round is loaded before the shift rows, so
+        // its round trip to L2 overlaps theirs (the guard keeps an empty
+        // leftover list from reading past the table's end)
+        const uint2* le = g.tko_left + g.tko_loff[c];
+        const int lc = g.tko_loff[c + 1] - g.tko_loff[c];
+        uint2 e0[16];
+        if (lc > 0) {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) e0[u] = le[min(u * 64 + lane_id(), lc - 1)];
+        } else {
+#pragma unroll
+            for (int u = 0; u < 16; ++u) e0[u] = make_uint2(0u, 0u);
+        }
+        if (W <= 64 * 20)
+            tko_groups<20>(g, c, cur, fut);  // one chunk: every load of a row in flight at once
+        else
+            tko_groups<8>(g, c, cur, fut);
         wave_sync();
-        scatter<CAP, false, 8>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, fl, 0);
+        scatter_pre<16>(e0, le, lc, cur, fut);
         *fut_sparse = false;
         return -1;
     }
@@ -1549,8 +1584,12 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
             if (msteps[m]) atomicAdd(&a.method_steps[m], (unsigned long long)msteps[m]);
 }
 
+// Two waves per SIMD: large rule sets fill the LDS with eight waves per CU,
+// and the register budget that leaves pays for the TKO step's one-chunk
+// rows (20 words per lane in flight) and 16-edge scatter rounds.
 template <int WPB, int CAP, int MODE>
-__global__ __launch_bounds__(64 * WPB) void engine_kernel(DevGraph g, ScanArgs a) {
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(1, 2))) void engine_kernel(DevGraph g,
+                                                                                                     ScanArgs a) {
     // the per-byte statistics stay in device memory (scalar loads): a 4 KB
     // copy per workgroup in LDS would cost large rule sets a workgroup per CU
     extern __shared__ uint64_t lds[];

@@ -44,7 +44,7 @@ EXPORTED = [
     "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
     "ita_selector_forward", "ita_serve_trace", "ita_stream_create", "ita_stream_reset", "ita_stream_scan",
     "ita_stream_destroy", "ita_last_results", "ita_scan_device_noalloc", "ita_set_policy", "ita_serve_ring",
-    "ita_last_times",
+    "ita_last_times", "ita_set_tko_threshold",
 ]
 
 _LIB = None
@@ -77,6 +77,7 @@ def lib():
             "ita_last_times": (i32, [vp, vp]),
             "ita_set_variant": (i32, [vp, i32, i32, i32, i32]),
             "ita_get_variant": (i32, [vp, vp, vp, vp, vp]),
+            "ita_set_tko_threshold": (i32, [vp, i32]),
             "ita_set_selector": (i32, [vp, vp, i32, i32]),
             "ita_set_policy": (i32, [vp, vp, i32]),
             "ita_stream_create": (i32, [vp, i32, ctypes.POINTER(vp)]),
@@ -230,6 +231,12 @@ class Scratch:
     def set_variant(self, wpb: int = 2, cap: int = 512, plain_bs: int = 64, store: int = 0):
         """store 0: LDS bitmaps; 1: hashed active lists (AS and learned modes)."""
         _err(lib().ita_set_variant(self._h, wpb, cap, plain_bs, store))
+
+    def set_tko_threshold(self, threshold: int):
+        """TKO shift-group threshold: -2 auto (W/2, the default), -1 the
+        reference's state_count/16, >= 0 that count.  Before alloc_scratch."""
+        _err(lib().ita_set_tko_threshold(self._h, threshold))
+        return self
 
     def variant(self):
         """(wpb, cap, plain_bs, store)"""

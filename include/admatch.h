@@ -154,8 +154,10 @@ int64_t ita_last_results(ita_handle* h, int32_t* out_ids, int64_t cap);
 int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3);
 
 /* Times (ms) of the handle's last scan: [0] the launch sequence (as
- * ita_last_stats), [1] the engine kernel alone (hipEvents around it, also
- * inside a replayed graph; 0 for the plain kernel), and for the last
+ * ita_last_stats), [1] the engine kernel alone (hipEvents around it; a
+ * graph-captured scan replays its prefix kernels as a graph and launches the
+ * engine kernel after it, so this holds with use_graph too; 0 for the plain
+ * kernel), and for the last
  * host-strings scan [2] device buffer growth, [3] host-to-device copies,
  * [4] result copies, [5] the host's accept-map lookup. */
 int ita_last_times(ita_handle* h, float* out6);
@@ -167,6 +169,17 @@ int ita_last_times(ita_handle* h, float* out6);
  * string independent of S) and the S-bit bitmaps in device memory, used by
  * dense steps only.  The other modes always use LDS bitmaps. */
 int ita_set_variant(ita_handle* h, int32_t wpb, int32_t cap, int32_t plain_bs, int32_t store);
+
+/* Grouping threshold of the TKO shift groups (src/transition_graph.cpp:298-357
+ * fixes it at state_count/16): an offset group of a byte becomes a shift row
+ * when it holds more than `threshold` transitions, the rest are scattered as
+ * single edges.  -1 = the reference's state_count/16, -2 = auto (the
+ * default): W/2 with W = ceil(state_count/64), since a shift row touches W
+ * words at about the cost of W/2 scattered edges; >= 0 = that count.  The
+ * partition changes only how TKO steps run, never results.  Call before
+ * ita_alloc (ITA_ERR_STATE after it); ita_info's [5] reports the count in
+ * use. */
+int ita_set_tko_threshold(ita_handle* h, int32_t threshold);
 int ita_get_variant(ita_handle* h, int32_t* wpb, int32_t* cap, int32_t* plain_bs, int32_t* store);
 
 /* Learned step selector: 5,443 packed float weights (layout in

@@ -131,7 +131,7 @@ class OracleNFA:
         return [rules[i * per_cap: i * per_cap + int(counts[i])].tolist() for i in range(n)]
 
 
-def tko_partition(nfa: OracleNFA, max_k: int = 1024):
+def tko_partition(nfa: OracleNFA, max_k: int = 1024, threshold: int = -1):
     """Restatement of the TKO table build, src/transition_graph.cpp:197-245,298-357.
 
     Per byte symbol c, the transitions on c plus one (s, s) per persistent
@@ -143,6 +143,9 @@ def tko_partition(nfa: OracleNFA, max_k: int = 1024):
     list.  Ties in size are ranked by ascending offset (the reference's
     std::sort leaves tie order unspecified).
 
+    threshold >= 0 replaces state_count/16 (the engine's tuned partitions;
+    the grouping rule is the same).
+
     Returns (k_per_symbol[257], {c: [(offset, sorted source array)]},
     leftover {c: sorted unique (src, dst) array}).
     """
@@ -151,7 +154,7 @@ def tko_partition(nfa: OracleNFA, max_k: int = 1024):
     _, persis, _ = nfa.vectors()
     bits = np.unpackbits(persis.view(np.uint8), bitorder="little")[: nfa.S]
     pstates = np.nonzero(bits)[0].astype(np.int64)
-    thr = nfa.S // 16
+    thr = nfa.S // 16 if threshold < 0 else threshold
     k_per = np.zeros(257, dtype=np.int64)
     groups, leftover = {}, {}
     for c in range(257):

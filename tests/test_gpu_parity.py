@@ -53,6 +53,23 @@ def test_crs_groups(gold_dir, group, name, flags):
     assert got == exp, first_diff(got, exp)
 
 
+@pytest.mark.parametrize("tko", [-1, 8])
+@pytest.mark.parametrize("group", ["g7", "g9"])
+def test_crs_tko_thresholds(gold_dir, group, tko):
+    # the TKO partition decides how steps run, never results: the reference's
+    # state_count/16 (-1) and a low count (8: up to 149 rows per byte, most
+    # with negative offsets); the default (W/2) runs in every other test
+    path = golden_nfa(f"crs_{group}.nfa.gz", gold_dir)
+    buf, off = golden_inputs("http_inputs.npz")
+    exp = golden_json("crs_expected.json")[group]
+    for flags in (E.TKO_KERNEL, E.AD_MATCHER):
+        print(f"[case] crs {group} tko threshold {tko} flags {flags}", flush=True)
+        s = E.Scratch(flags, path).set_tko_threshold(tko).alloc_scratch()
+        got = E.unpack(*s.batched_scan_packed(buf, off))
+        s.close()
+        assert got == exp, first_diff(got, exp)
+
+
 @pytest.mark.parametrize("name,flags", PATHS, ids=[p[0] for p in PATHS])
 def test_synthetic_small_vs_re(gold_dir, name, flags):
     path = golden_nfa("synth_small.nfa.gz", gold_dir)

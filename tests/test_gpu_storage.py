@@ -68,6 +68,23 @@ def test_storages_vs_oracle(tmp_path, kind, seed):
     s.close()
 
 
+@pytest.mark.parametrize("tko", [-1, 0])
+@pytest.mark.parametrize("kind,seed", [("shift", 1), ("burst", 0)])
+def test_tko_thresholds_vs_oracle(tmp_path, kind, seed, tko):
+    # every offset group a shift row (0) and the reference's partition (-1),
+    # through every method that runs TKO steps, with both storages' dense paths
+    path, strs, exp = make_case(tmp_path, kind, seed)
+    buf, off = E.pack_strings(strs)
+    s = E.Scratch(E.AD_MATCHER, path).set_tko_threshold(tko).alloc_scratch()
+    d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+    for mode in (E.MODE_TKO, E.MODE_ADMATCH, E.MODE_HEURISTIC):
+        print(f"[case] {kind}{seed} tko threshold {tko} mode={E.MODE_NAMES[mode]}", flush=True)
+        s.scan_device(d_buf.ptr, d_off.ptr, len(strs), E.KIND_ENGINE, mode)
+        got = E.unpack(*s.collect(len(strs)))
+        assert got == exp, (tko, mode, first_diff(got, exp))
+    s.close()
+
+
 def test_hashed_graph_replay_and_growth(tmp_path):
     # captured launches keep their HBM bitmaps valid when a larger batch
     # grows the buffer (the graphs are dropped and captured again)

@@ -21,9 +21,12 @@ def to64(words32: np.ndarray) -> np.ndarray:
     return w.view(np.uint64)
 
 
-def same_as_oracle(path: str):
+def same_as_oracle(path: str, tko: int = -1):
+    """The product's tables against the oracle's; tko is the TKO grouping
+    threshold setting (-1: the reference's state_count/16)."""
     o = oracle.OracleNFA(path)
     s = E.ItaScratch(E.INFA_KERNEL, path)
+    s.set_tko_threshold(tko)
     assert (s.table(0).astype(np.int64) == o.offsets()).all()
     assert (s.table(1).reshape(-1, 2) == o.transitions()).all()
     v = s.table(9).reshape(3, -1)
@@ -46,8 +49,10 @@ def test_parse_random(tmp_path, seed):
     same_as_oracle(str(p))
 
 
-def check_tko(s, o):
-    k_per, groups, leftover = oracle.tko_partition(o)
+def check_tko(s, o, threshold: int = -1):
+    want_thr = o.S // 16 if threshold < 0 else threshold
+    assert s.info()["tko_threshold"] == want_thr
+    k_per, groups, leftover = oracle.tko_partition(o, threshold=want_thr)
     koff = s.table(2)
     assert (np.diff(koff) == k_per).all()
     shifts = s.table(3)
@@ -55,7 +60,6 @@ def check_tko(s, o):
     lim = s.table(4).reshape(-1, W) if shifts.size else np.zeros((0, W), np.uint64)
     loff = s.table(5)
     left = s.table(6).reshape(-1, 2)
-    assert s.info()["tko_threshold"] == o.S // 16
     # groups in symbol order, ranked within a symbol: offsets and source masks
     flat = [g for c in range(257) for g in groups.get(c, [])]
     assert shifts.tolist() == [d for d, _ in flat]
@@ -74,6 +78,41 @@ def check_tko(s, o):
 def test_tko_partition_fixtures(gold_dir, name):
     s, o = same_as_oracle(golden_nfa(name, gold_dir))
     check_tko(s, o)
+
+
+@pytest.mark.parametrize("name", ["crs_g6.nfa.gz", "crs_g9.nfa.gz", "synth_small.nfa.gz"])
+def test_tko_partition_tuned(gold_dir, name):
+    # the engine's default partition: offset groups above W/2 transitions
+    s, o = same_as_oracle(golden_nfa(name, gold_dir), tko=-2)
+    W = s.info()["words"]
+    check_tko(s, o, threshold=W // 2)
+    s.close()
+    # an explicit count, and the setting is refused once the tables are on the device
+    s, o = same_as_oracle(golden_nfa(name, gold_dir), tko=7)
+    check_tko(s, o, threshold=7)
+    with pytest.raises(E.AdmatchError):
+        s.set_tko_threshold(-3)
+    s.close()
+
+
+def test_tko_partition_threshold_closed_forms(tmp_path):
+    # the grouping rule against hand counts: byte 97 has offset +1 on 5
+    # states, offset +2 on 3, offset -4 on 1; one persistent state adds a
+    # (s, s) group of 1 on every byte
+    lines = ["12"] + [f"{i} -> {i + 1} : 97" for i in range(5)] + [f"{i} -> {i + 2} : 97" for i in (6, 7, 8)]
+    lines += ["10 -> 6 : 97", "11 -> 11 : 0|255", "0 : initial", "5 : accepting 1"]
+    p = tmp_path / "t.nfa"
+    p.write_text("\n".join(lines) + "\n")
+    o = oracle.OracleNFA(str(p))
+    for thr, want in [(0, [(1, 5), (2, 3), (-4, 1), (0, 1)]), (1, [(1, 5), (2, 3)]), (2, [(1, 5), (2, 3)]),
+                      (3, [(1, 5)]), (5, [])]:
+        k, groups, left = oracle.tko_partition(o, threshold=thr)
+        got = [(d, len(src)) for d, src in groups.get(97, [])]
+        assert got == want, thr
+        assert sum(n for _, n in got) + left[97].shape[0] == 10  # 9 edges + the self-loop
+        s, o2 = same_as_oracle(str(p), tko=thr)
+        check_tko(s, o2, threshold=thr)
+        s.close()
 
 
 @pytest.mark.parametrize("name,k_max,k_sum,neg,neg_unaligned", [
