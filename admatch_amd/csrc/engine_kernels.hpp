@@ -1096,17 +1096,23 @@ __device__ bool serve_wait(const ScanArgs& a, int t) {
     int go = 0;
     if (lane_id() == 0) {
         const uint64_t t0 = wall_clock64();
+        // Polls are relaxed loads and the acquire is one fence once the
+        // ticket is admitted: an acquire load at agent or system scope
+        // invalidates caches, and hundreds of waiting waves polling with one
+        // would keep evicting the table rows of the waves that are scanning.
         for (;;) {
-            const int m = __hip_atomic_load(&a.serve_mirror[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            const int m = __hip_atomic_load(&a.serve_mirror[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (m > t) {
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 go = 1;
                 break;
             }
             if (__hip_atomic_load(&a.serve_mirror[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
             if (m == t) {
                 // the lowest unpublished ticket: read the host's counter
-                const int p = __hip_atomic_load(a.serve_published, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                const int p = __hip_atomic_load(a.serve_published, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 if (p > t) {
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
                     atomicMax(&a.serve_mirror[0], p);
                     go = 1;
                     break;
