@@ -15,11 +15,15 @@ struct DevGraph {
     const int32_t* sym_off; // [258] iNFA CSR offsets
     const uint2* list;      // (src,dst) per symbol, sorted
     const int32_t* tko_koff;  // [258]
-    const int32_t* tko_shift; // [K]
-    const uint64_t* tko_lim;  // distinct source-mask rows, W words each
-    const int32_t* tko_row;   // [K] row of group i in tko_lim
+    const int2* tko_grp;      // [K] group i: shift d and its row in tko_lim
+    const uint64_t* tko_lim;  // distinct source-mask rows: word j of row r at r * tko_stride + kTkoPad + j
+    int32_t tko_stride;       // zero outside [0, W), readable for j in [-kTkoPad, tko_stride - kTkoPad)
     const int32_t* tko_loff;  // [258]
     const uint2* tko_left;    // leftover (src,dst)
+    // [256][64] per byte, one int2 per lane: lanes 0..59 the (shift, row) of
+    // the byte's first 60 groups, lane 60 its groups [x, y), lane 61 its
+    // leftover edges [x, y)
+    const int2* tko_rec;
     const uint32_t* as_tab;   // [256*S] AS rows, byte major (null: AS unavailable)
     const uint32_t* as_multi; // destinations of multi-entry rows
     const uint32_t* alw_dst;  // always-active rows per byte
@@ -110,6 +114,14 @@ constexpr int32_t kModeLean = 0x100;
 
 enum Method : int32_t { M_INFA = 0, M_TKO = 1, M_AS = 2 };
 
+constexpr int kTkoRecGroups = 60;  // groups held in a byte's TKO record
+// TKO row passes: one pass of up to kTkoOnePass 64-word chunks per lane when
+// the state vector fits, else passes of kTkoPassChunks chunks; rows padded
+// with kTkoPad zero words on each side.
+constexpr int kTkoOnePass = 18;
+constexpr int kTkoPassChunks = 8;
+constexpr int kTkoPad = 64;
+
 constexpr int kFeatures = 16;
 constexpr int kPolicyTableWords = 512;  // 16 buckets x 2 list states x 256 bytes x 2 bits
 constexpr int kHidden = 64;
@@ -140,9 +152,18 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
+// Cross-lane moves below are DPP modifiers of VALU moves (no LDS round
+// trip): quad permutes, row (16-lane) rotates and shifts.  The whole wave
+// must be active.
+// Sum over the 64 lanes (the same value in every lane).
 __device__ __forceinline__ int wave_sum(int v) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x124, 0xf, 0xf, false);  // row_ror:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x128, 0xf, 0xf, false);  // row_ror:8
+    // every lane holds its row's sum
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+           __builtin_amdgcn_readlane(v, 48);
 }
 __device__ __forceinline__ float wave_sum_f(float v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -150,12 +171,17 @@ __device__ __forceinline__ float wave_sum_f(float v) {
 }
 // inclusive prefix sum across the 64 lanes
 __device__ __forceinline__ int wave_scan(int v) {
+    // within each row of 16 (a lane whose source is before its row's start reads 0)
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, true);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, true);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, true);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, true);  // row_shr:8
+    // then the totals of the rows below
+    const int r0 = __builtin_amdgcn_readlane(v, 15);
+    const int r1 = r0 + __builtin_amdgcn_readlane(v, 31);
+    const int r2 = r1 + __builtin_amdgcn_readlane(v, 47);
     const int l = lane_id();
-    for (int o = 1; o < 64; o <<= 1) {
-        int t = __shfl_up(v, o, 64);
-        if (l >= o) v += t;
-    }
-    return v;
+    return v + (l < 16 ? 0 : l < 32 ? r0 : l < 48 ? r1 : r2);
 }
 __device__ __forceinline__ bool test_bit(const uint64_t* v, uint32_t i) { return (v[i >> 6] >> (i & 63)) & 1ull; }
 

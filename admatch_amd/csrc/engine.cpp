@@ -319,7 +319,6 @@ int Engine::upload(std::string& err) {
     ADM_PUT(soff, sym_off)
     ADM_PUT(g_.list, list)
     ADM_PUT(g_.tko_koff, tko_koff)
-    ADM_PUT(g_.tko_shift, tko_shift)
     {
         // each distinct source-mask row once (groups of different bytes
         // often share one: the persistent states, a class run's chain), so
@@ -347,8 +346,32 @@ int Engine::upload(std::string& err) {
             row_of[i] = found;
         }
         tko_rows_ = int64_t(rows.size() / std::max<size_t>(W, 1));
-        ADM_PUT(rows, tko_lim)
-        ADM_PUT(row_of, tko_row)
+        // each row stored with kTkoPad zero words before it and zeros after
+        // it up to the last word a shift pass can read, so the kernels read
+        // shifted rows without bounds checks
+        const size_t pass = size_t(64 * kTkoPassChunks);
+        size_t cover = (W + pass - 1) / pass * pass;  // passes of kTkoPassChunks chunks
+        if (W <= size_t(64 * kTkoOnePass)) cover = std::max(cover, size_t(64 * kTkoOnePass));  // one pass
+        const size_t stride = size_t(kTkoPad) + cover + size_t(kTkoPad);
+        std::vector<uint64_t> padded(std::max<size_t>(size_t(tko_rows_), 1) * stride, 0ull);
+        for (int64_t r = 0; r < tko_rows_; ++r)
+            std::copy(rows.begin() + r * W, rows.begin() + (r + 1) * W, padded.begin() + r * stride + kTkoPad);
+        dev_.tko_stride = int32_t(stride);
+        ADM_PUT(padded, tko_lim)
+        // (shift, row) per group: a step reads its byte's groups with one load
+        std::vector<int2> grp(std::max<size_t>(K, 1), make_int2(0, 0));
+        for (size_t i = 0; i < K; ++i) grp[i] = make_int2(g_.tko_shift[i], row_of[i]);
+        ADM_PUT(grp, tko_grp)
+        // one 512-byte record per byte, read by a wave with one load
+        std::vector<int2> rec(256 * 64, make_int2(0, 0));
+        for (int c = 0; c < 256; ++c) {
+            int2* r = rec.data() + c * 64;
+            const int kb = g_.tko_koff[c], ke = g_.tko_koff[c + 1];
+            for (int i = 0; i < std::min(ke - kb, kTkoRecGroups); ++i) r[i] = grp[size_t(kb + i)];
+            r[kTkoRecGroups] = make_int2(kb, ke);
+            r[kTkoRecGroups + 1] = make_int2(g_.tko_loff[c], g_.tko_loff[c + 1]);
+        }
+        ADM_PUT(rec, tko_rec)
     }
     ADM_PUT(g_.tko_loff, tko_loff)
     ADM_PUT(g_.tko_left, tko_left)

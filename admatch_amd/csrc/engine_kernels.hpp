@@ -286,60 +286,121 @@ __device__ __forceinline__ void scatter_pre(const uint2 (&e0)[U], const uint2* e
     }
 }
 
-// TKO shift groups of symbol c: fut[j] = OR over groups i of shift(lim_i &
-// cur, d_i)[j], fut zero on entry.  Lane l owns word j = j0 + 64u + l of U
-// consecutive 64-word chunks and accumulates every group in registers, so
-// each word is stored once and no atomics are needed; the U chunks' loads
-// are issued together.  Shifts are split as d = 64q + r with 0 <= r < 64
-// (floor division): word j takes source word j-q shifted left by r and,
-// when r != 0, source word j-q-1 shifted right by 64-r (the r == 0 case
-// never shifts by 64).  Word j-q-1 is the lane below's word j-q, taken with
-// a lane shuffle; lane 0 takes it from lane 63 of the previous chunk, or
-// loads it for the first chunk.
+// A byte's TKO record as one wave holds it (one int2 per lane): lanes
+// 0..kTkoRecGroups-1 the (shift, row) of its first groups, lane
+// kTkoRecGroups its groups [kb, ke), the next lane its leftover edges [lb, le).
+struct TkoRec {
+    int2 v;
+    __device__ __forceinline__ int kb() const { return __builtin_amdgcn_readlane(v.x, kTkoRecGroups); }
+    __device__ __forceinline__ int ke() const { return __builtin_amdgcn_readlane(v.y, kTkoRecGroups); }
+    __device__ __forceinline__ int lb() const { return __builtin_amdgcn_readlane(v.x, kTkoRecGroups + 1); }
+    __device__ __forceinline__ int le() const { return __builtin_amdgcn_readlane(v.y, kTkoRecGroups + 1); }
+};
+
+// Makes the compiler wait for a loaded value at this point (an empty asm
+// statement that reads it).
+__device__ __forceinline__ void wait_ready(int2 v) { asm volatile("" ::"v"(v.x), "v"(v.y)); }
+
+__device__ __forceinline__ TkoRec tko_rec(const DevGraph& g, int c) {
+    return TkoRec{g.tko_rec[(c << 6) + lane_id()]};
+}
+
+// x of lane (l - 1) mod 64 in lane l: a DPP row rotate over the whole wave
+// (wave_ror:1), a VALU move that needs no LDS round trip.
+__device__ __forceinline__ uint64_t wave_ror1(uint64_t x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, int(uint32_t(x)), 0x13C, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, int(uint32_t(x >> 32)), 0x13C, 0xf, 0xf, false);
+    return uint64_t(uint32_t(hi)) << 32 | uint32_t(lo);
+}
+
+// TKO shift groups of one byte (record tk): fut[j] = OR over groups i of
+// shift(lim_i & cur, d_i)[j], fut zero on entry.  Lane l owns word j = j0 +
+// 64u + l of U consecutive 64-word chunks and accumulates every group in
+// registers, so each word is stored once and no atomics are needed; the U
+// chunks' loads are issued together.  Shifts are split as d = 64q + r with
+// 0 <= r < 64 (floor division): word j takes source word j-q shifted left by
+// r and, when r != 0, source word j-q-1 shifted right by 64-r (the r == 0
+// case never shifts by 64).  Word j-q-1 is the lane below's word j-q, moved
+// up one lane by a wave rotate; lane 0 takes it from lane 63 of the previous
+// chunk, or loads it for the first chunk.  The rows are zero-padded by
+// kTkoPad words on each side, so for |q| within the padding the row words
+// are read unchecked and the cur word at a clamped index (the zero row word
+// masks it); the funnel shifts are two 32-bit alignbits per word.
 template <int U>
-__device__ __forceinline__ void tko_groups(const DevGraph& g, int c, const uint64_t* cur, uint64_t* fut) {
+__device__ __forceinline__ void tko_groups(const DevGraph& g, const TkoRec& tk, const uint64_t* cur, uint64_t* fut) {
     const int W = g.W;
-    const int kb = g.tko_koff[c], ke = g.tko_koff[c + 1];
+    const int kb = tk.kb(), ke = tk.ke();
     if (kb == ke) return;
     const int l = lane_id();
+    const bool l0 = l == 0;
+    auto lo = [](uint64_t v) { return uint32_t(v); };
+    auto hi = [](uint64_t v) { return uint32_t(v >> 32); };
     for (int j0 = 0; j0 < W; j0 += 64 * U) {
         uint64_t acc[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) acc[u] = 0;
-        for (int i = kb; i < ke; ++i) {
-            const int d = g.tko_shift[i];
+        for (int i = 0; i < ke - kb; ++i) {
+            // the record's groups come from its lanes, later ones from memory
+            int d, row;
+            if (i < kTkoRecGroups) {
+                d = __builtin_amdgcn_readlane(tk.v.x, i);
+                row = __builtin_amdgcn_readlane(tk.v.y, i);
+            } else {
+                const int2 t = g.tko_grp[kb + i];
+                d = uniform(t.x);
+                row = uniform(t.y);
+            }
             const int q = d >> 6;  // arithmetic shift: floor(d / 64)
             const int r = d & 63;
-            const uint64_t* lim = g.tko_lim + int64_t(g.tko_row[i]) * W;
-            // clamped, unconditional loads (see scatter), issued together
+            const uint64_t* lim = g.tko_lim + (int64_t(row) * g.tko_stride + kTkoPad);
             uint64_t a[U];
+            uint64_t prev;  // lane 0's word below the first chunk (the same address in every lane)
+            if (q >= -kTkoPad && q < kTkoPad) {
+                const uint64_t* lrow = lim + (j0 - q);
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int s0 = j0 + 64 * u + l - q;
-                const int sc = min(max(s0, 0), W - 1);
-                const uint64_t v = lim[sc] & cur[sc];
-                a[u] = s0 == sc ? v : 0ull;
-            }
-            if (r != 0) {
-                // the word below the first chunk (the same address in every lane)
+                for (int u = 0; u < U; ++u) a[u] = lrow[64 * u + l] & cur[min(max(j0 + 64 * u + l - q, 0), W - 1)];
+                prev = lrow[-1] & cur[min(max(j0 - q - 1, 0), W - 1)];
+            } else {
+                // far shifts: clamped reads, words outside [0, W) dropped
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int s0 = j0 + 64 * u + l - q;
+                    const int sc = min(max(s0, 0), W - 1);
+                    const uint64_t v = lim[sc] & cur[sc];
+                    a[u] = s0 == sc ? v : 0ull;
+                }
                 const int s1 = j0 - q - 1;
                 const int sc1 = min(max(s1, 0), W - 1);
                 const uint64_t v1 = lim[sc1] & cur[sc1];
-                const uint64_t first = s1 == sc1 ? v1 : 0ull;
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const uint64_t up = uint64_t(__shfl_up(static_cast<long long>(a[u]), 1, 64));
-                    // lane 0: lane 63 of the previous chunk (a select, not a branch)
-                    const uint64_t fix = u == 0 ? first
-                                                : uint64_t(uint32_t(__builtin_amdgcn_readlane(int(a[u - 1] >> 32), 63)))
-                                                          << 32 |
-                                                      uint32_t(__builtin_amdgcn_readlane(int(a[u - 1]), 63));
-                    const uint64_t b = l == 0 ? fix : up;
-                    acc[u] |= (a[u] << r) | (b >> (64 - r));
-                }
-            } else {
+                prev = s1 == sc1 ? v1 : 0ull;
+            }
+            if (r == 0) {
 #pragma unroll
                 for (int u = 0; u < U; ++u) acc[u] |= a[u];
+            } else if (r <= 32) {
+                // (a << r) | (b >> (64 - r)) needs only the high half of the word below b
+                const uint32_t s = uint32_t(32 - r);
+                uint32_t bprev = hi(prev);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t ah = hi(a[u]), al = lo(a[u]);
+                    const uint32_t rot = uint32_t(__builtin_amdgcn_update_dpp(0, int(ah), 0x13C, 0xf, 0xf, false));
+                    const uint32_t bh = l0 ? bprev : rot;
+                    acc[u] |= uint64_t(__builtin_amdgcn_alignbit(ah, al, s)) << 32 | __builtin_amdgcn_alignbit(al, bh, s);
+                    bprev = rot;  // lane 0 of the next chunk: lane 63 of this one
+                }
+            } else {
+                const uint32_t t = uint32_t(64 - r);
+                uint64_t bprev = prev;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint64_t rot = wave_ror1(a[u]);
+                    const uint64_t b = l0 ? bprev : rot;
+                    const uint32_t bh = hi(b);
+                    acc[u] |= uint64_t(__builtin_amdgcn_alignbit(lo(a[u]), bh, t)) << 32 |
+                              __builtin_amdgcn_alignbit(bh, lo(b), t);
+                    bprev = rot;
+                }
             }
         }
 #pragma unroll
@@ -393,16 +454,21 @@ struct RowSlice {
 // byte): their rows on c are the precomputed two-byte rows (pair, merged
 // here when use_pair) and the always-active rows of c, which become the
 // next step's implicit set.  Always-active states (bit 31) are therefore
-// neither gathered nor stored.
+// neither gathered nor stored.  clr (or null): the bitmap of the listed
+// states, whose words are cleared as the list is read.
 template <int CAP, typename SET>
 __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_pair, const uint32_t* cl, int n_cur,
-                       const SET& fut, uint32_t* fl) {
+                       const SET& fut, uint32_t* fl, uint64_t* clr) {
     // state-major table: entry (s, c) at s*256 + c
     const uint32_t* tab = g.as_tab + c;
     // the first chunk's table gathers are in flight while the two-byte rows
     // are merged
     const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
     const uint32_t e0 = (sv0 & kTag) ? 0u : tab[(sv0 & kIdMask) << 8];
+    if (clr != nullptr) {
+        if (lane_id() < n_cur) clr[(sv0 & kIdMask) >> 6] = 0ull;
+        for (int k = 64 + lane_id(); k < n_cur; k += 64) clr[(cl[k] & kIdMask) >> 6] = 0ull;
+    }
     // common case: one chunk of states, one of two-byte row entries, rows of
     // at most two destinations (a persistent state's row is itself plus the
     // state it advances to): every lane's candidates in one pass
@@ -706,16 +772,17 @@ __device__ __forceinline__ void persist_dense(const uint64_t* cur, const uint64_
 // Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
 // Returns the fut count (list valid iff *fut_sparse).
 template <int CAP>
-__device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, const uint32_t* cl, int n_cur,
-                          bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse) {
+__device__ int run_method(const DevGraph& g, int m, int c, const TkoRec& tk, const uint64_t* cur, const uint32_t* cl,
+                          int n_cur, bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse) {
     int n = 0;
     const int W = g.W;
     if (m == M_TKO) {
         // the leftover edges' first round is loaded before the shift rows, so
         // its round trip to L2 overlaps theirs (the guard keeps an empty
         // leftover list from reading past the table's end)
-        const uint2* le = g.tko_left + g.tko_loff[c];
-        const int lc = g.tko_loff[c + 1] - g.tko_loff[c];
+        const int lb = tk.lb();
+        const uint2* le = g.tko_left + lb;
+        const int lc = tk.le() - lb;
         uint2 e0[16];
         if (lc > 0) {
 #pragma unroll
@@ -724,10 +791,14 @@ __device__ int run_method(const DevGraph& g, int m, int c, const uint64_t* cur, 
 #pragma unroll
             for (int u = 0; u < 16; ++u) e0[u] = make_uint2(0u, 0u);
         }
-        if (W <= 64 * 20)
-            tko_groups<20>(g, c, cur, fut);  // one chunk: every load of a row in flight at once
+        // one pass when the vector fits in kTkoOnePass chunks per lane (every
+        // load of a row in flight at once); two sizes keep the kernel's code
+        // small
+        const int nch = (W + 63) >> 6;
+        if (nch > kTkoPassChunks && nch <= kTkoOnePass)
+            tko_groups<kTkoOnePass>(g, tk, cur, fut);
         else
-            tko_groups<8>(g, c, cur, fut);
+            tko_groups<kTkoPassChunks>(g, tk, cur, fut);
         wave_sync();
         scatter_pre<16>(e0, le, lc, cur, fut);
         *fut_sparse = false;
@@ -918,6 +989,12 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
     };
     RowSlice pair{0u, 0, 0};
     const bool can_as = g.as_tab != nullptr;
+    // After a step that leaves a dense set, the next byte's TKO record is
+    // loaded (tk_next, valid when tk_ready) for a TKO step to use; other TKO
+    // steps load their own.
+    constexpr bool kTko = MODE != MODE_INFA && MODE != MODE_AS;
+    TkoRec tk_next{make_int2(0, 0)};
+    bool tk_ready = false;
 
     StepCtx ctx;
     ctx.len = L;
@@ -927,11 +1004,24 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
     for (int b = 0; b < L; ++b) {
         // the next byte is read one step ahead (two-byte rows, word boundaries)
         const int c = c_next;
+        TkoRec tk = tk_next;
+        // the record is waited for here, before this step issues loads of
+        // its own (loads complete in order: a wait for it later would also
+        // wait for those)
+        if (kTko) wait_ready(tk.v);
+        const bool tk_ok = tk_ready;
         RowSlice pair_next{0u, 0, 0};
         if (b + 1 < L) {
             c_next = win.at(b + 1);
             if (have_pair) pair_next = load_pair(c, c_next);
         }
+        auto tk_of = [&]() {
+            if (!tk_ok) {
+                tk = tko_rec(g, c);
+                wait_ready(tk.v);
+            }
+            return tk;
+        };
         // active states including the implicit ones (selector features)
         auto total_active = [&]() {
             int n_tot = n_act;
@@ -958,13 +1048,13 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
         int next_imp = -1;
         if (!log_step) {
             if (m == M_AS) {
-                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl);
+                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl, cur);
                 fsp = n_fut <= CAP;
                 next_imp = c;
             } else {
                 if (MODE == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
                 materialize();
-                n_fut = run_method<CAP>(g, m, c, cur, cl, n_act, sparse, fut, fl, &fsp);
+                n_fut = run_method<CAP>(g, m, c, tk_of(), cur, cl, n_act, sparse, fut, fl, &fsp);
             }
         } else {
             // time every method on the same state: AS on the implicit form,
@@ -973,7 +1063,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
             if (sparse && can_as) {
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl);
+                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl, nullptr);
                 fsp = n_fut <= CAP;
                 wave_sync();
                 __builtin_amdgcn_s_waitcnt(0);
@@ -992,7 +1082,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
             for (int mm = M_TKO; mm >= M_INFA; --mm) {
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = run_method<CAP>(g, mm, c, cur, cl, n_act, sparse, fut, fl, &fsp);
+                n_fut = run_method<CAP>(g, mm, c, tko_rec(g, c), cur, cl, n_act, sparse, fut, fl, &fsp);
                 wave_sync();
                 if (!fsp) {  // a bitmap-only result pays for its list rebuild
                     n_fut = uniform(rebuild_list<CAP>(fut, W, fl));
@@ -1021,8 +1111,11 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
             }
             m = M_INFA;
         }
+        tk_ready = kTko && !fsp && b + 1 < L;
+        if (tk_ready) tk_next = tko_rec(g, c_next);
         wave_sync();
-        clear_bits<CAP>(cur, cl, n_act, sparse, W);
+        // an AS step cleared the listed states' words as it read them
+        if (log_step || m != M_AS) clear_bits<CAP>(cur, cl, n_act, sparse, W);
         // swap
         uint64_t* tb = cur;
         cur = fut;
@@ -1193,9 +1286,10 @@ __device__ __forceinline__ void dense_sync() {
 // iNFA or TKO step on whole bitmaps (fut zero on entry).
 static inline __device__ void dense_method(const DevGraph& g, int m, int c, const uint64_t* cur, uint64_t* fut) {
     if (m == M_TKO) {
-        tko_groups<1>(g, c, cur, fut);
+        const TkoRec tk = tko_rec(g, c);
+        tko_groups<1>(g, tk, cur, fut);
         dense_sync();
-        scatter<64, false>(g.tko_left + g.tko_loff[c], g.tko_loff[c + 1] - g.tko_loff[c], cur, fut, nullptr, 0);
+        scatter<64, false>(g.tko_left + tk.lb(), tk.le() - tk.lb(), cur, fut, nullptr, 0);
     } else {
         for (int j = lane_id(); j < g.W; j += 64) fut[j] = cur[j] & g.persis[j];
         dense_sync();
@@ -1388,7 +1482,7 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, con
         int n_fut = 0;
         bool fsp = false;
         if (m == M_AS) {
-            n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, hf, fl);
+            n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, hf, fl, nullptr);
             wave_sync();
             if (n_fut <= CAP) {
                 fsp = true;
