@@ -111,6 +111,10 @@ enum SelectMode : int32_t {
 // boundary code and the path for missing rows are compiled out of the step
 // loop.
 constexpr int32_t kModeLean = 0x100;
+// LDS-storage kernels are also built with this bit added to the mode, for
+// state vectors of at most 64 * kTkoPassChunks words: a smaller TKO step and
+// up to four waves per SIMD.
+constexpr int32_t kModeNarrow = 0x200;
 
 enum Method : int32_t { M_INFA = 0, M_TKO = 1, M_AS = 2 };
 

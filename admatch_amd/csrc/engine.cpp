@@ -490,8 +490,7 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     if (prefix) ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
     if (!engine) return ADM_OK;
     const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
-    // lean hashed kernels for rule sets without boundaries and with two-byte rows
-    const int32_t kmode = (hs && g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
+    const int32_t kmode = engine_kmode(mode, hs);
     ADM_HIP(hipEventRecord(c.ev_k0, st));
     ADM_HIP(launch_engine(var_.wpb, var_.cap, kmode, hs, grid, lds, st, dev_, a));
     ADM_HIP(hipEventRecord(c.ev_k1, st));
@@ -542,6 +541,11 @@ int Engine::prepare_dense(Context& c, int32_t mode, std::string& err) {
     return c.ensure_dense(size_t(grid) * var_.wpb * 2 * size_t(g_.W), err);
 }
 
+int32_t Engine::engine_kmode(int32_t mode, bool hs) const {
+    if (hs) return (g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
+    return g_.W <= 64 * kTkoPassChunks ? (mode | kModeNarrow) : mode;
+}
+
 int Engine::engine_grid(int32_t mode, int32_t n, bool hs, int* grid, std::string& err) {
     const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
     if (lds > max_lds_) {
@@ -549,8 +553,7 @@ int Engine::engine_grid(int32_t mode, int32_t n, bool hs, int* grid, std::string
         return ADM_ERR_TOO_LARGE;
     }
     int blocks = 0;
-    const int32_t kmode = (hs && g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
-    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, kmode, hs, lds, &blocks));
+    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, engine_kmode(mode, hs), hs, lds, &blocks));
     if (blocks <= 0) {
         err = "engine variant does not fit on a CU";
         return ADM_ERR_TOO_LARGE;

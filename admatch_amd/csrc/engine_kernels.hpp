@@ -770,10 +770,13 @@ __device__ __forceinline__ void persist_dense(const uint64_t* cur, const uint64_
 }
 
 // Runs one step with method m from (cur, cl, n_cur, sparse) into fut/fl.
-// Returns the fut count (list valid iff *fut_sparse).
-template <int CAP>
+// Returns the fut count (list valid iff *fut_sparse).  NARROW: the kernel is
+// built for state vectors of at most 64 * kTkoPassChunks words and more waves
+// per SIMD, so the TKO step keeps fewer loads in flight.
+template <int CAP, bool NARROW>
 __device__ int run_method(const DevGraph& g, int m, int c, const TkoRec& tk, const uint64_t* cur, const uint32_t* cl,
                           int n_cur, bool sparse, uint64_t* fut, uint32_t* fl, bool* fut_sparse) {
+    constexpr int UE = NARROW ? 8 : 16;  // leftover edges per lane loaded with the record
     int n = 0;
     const int W = g.W;
     if (m == M_TKO) {
@@ -783,24 +786,24 @@ __device__ int run_method(const DevGraph& g, int m, int c, const TkoRec& tk, con
         const int lb = tk.lb();
         const uint2* le = g.tko_left + lb;
         const int lc = tk.le() - lb;
-        uint2 e0[16];
+        uint2 e0[UE];
         if (lc > 0) {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) e0[u] = le[min(u * 64 + lane_id(), lc - 1)];
+            for (int u = 0; u < UE; ++u) e0[u] = le[min(u * 64 + lane_id(), lc - 1)];
         } else {
 #pragma unroll
-            for (int u = 0; u < 16; ++u) e0[u] = make_uint2(0u, 0u);
+            for (int u = 0; u < UE; ++u) e0[u] = make_uint2(0u, 0u);
         }
         // one pass when the vector fits in kTkoOnePass chunks per lane (every
         // load of a row in flight at once); two sizes keep the kernel's code
         // small
         const int nch = (W + 63) >> 6;
-        if (nch > kTkoPassChunks && nch <= kTkoOnePass)
+        if (!NARROW && nch > kTkoPassChunks && nch <= kTkoOnePass)
             tko_groups<kTkoOnePass>(g, tk, cur, fut);
         else
             tko_groups<kTkoPassChunks>(g, tk, cur, fut);
         wave_sync();
-        scatter_pre<16>(e0, le, lc, cur, fut);
+        scatter_pre<UE>(e0, le, lc, cur, fut);
         *fut_sparse = false;
         return -1;
     }
@@ -842,7 +845,7 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
     }
 }
 
-template <int CAP, int MODE>
+template <int CAP, int MODE, bool NARROW>
 __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const uint8_t* x, int L, uint64_t* bitsA,
                             uint64_t* bitsB, uint32_t* listA, uint32_t* listB, int* msteps) {
     const int W = g.W;
@@ -1054,7 +1057,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
             } else {
                 if (MODE == MODE_LEARNED_AS && !sparse) m = dense_policy(a, n_act, c);
                 materialize();
-                n_fut = run_method<CAP>(g, m, c, tk_of(), cur, cl, n_act, sparse, fut, fl, &fsp);
+                n_fut = run_method<CAP, NARROW>(g, m, c, tk_of(), cur, cl, n_act, sparse, fut, fl, &fsp);
             }
         } else {
             // time every method on the same state: AS on the implicit form,
@@ -1082,7 +1085,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
             for (int mm = M_TKO; mm >= M_INFA; --mm) {
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = run_method<CAP>(g, mm, c, tko_rec(g, c), cur, cl, n_act, sparse, fut, fl, &fsp);
+                n_fut = run_method<CAP, NARROW>(g, mm, c, tko_rec(g, c), cur, cl, n_act, sparse, fut, fl, &fsp);
                 wave_sync();
                 if (!fsp) {  // a bitmap-only result pays for its list rebuild
                     n_fut = uniform(rebuild_list<CAP>(fut, W, fl));
@@ -1685,11 +1688,15 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
 }
 
 // Two waves per SIMD: large rule sets fill the LDS with eight waves per CU,
-// and the register budget that leaves pays for the TKO step's one-chunk
-// rows (20 words per lane in flight) and 16-edge scatter rounds.
+// and the register budget that leaves pays for the TKO step's one-pass rows
+// (kTkoOnePass words per lane in flight) and 16-edge scatter rounds.
+// (MODE & kModeNarrow: state vectors of at most 64 * kTkoPassChunks words;
+// the smaller TKO step lets four waves per SIMD fit in the registers)
 template <int WPB, int CAP, int MODE>
-__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(1, 2))) void engine_kernel(DevGraph g,
-                                                                                                     ScanArgs a) {
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu((MODE & kModeNarrow) ? 4 : 1, (MODE & kModeNarrow) ? 4 : 2))) void
+engine_kernel(DevGraph g, ScanArgs a) {
+    constexpr int M = MODE & ~kModeNarrow;
+    constexpr bool kNarrow = (MODE & kModeNarrow) != 0;
     // the per-byte statistics stay in device memory (scalar loads): a 4 KB
     // copy per workgroup in LDS would cost large rule sets a workgroup per CU
     extern __shared__ uint64_t lds[];
@@ -1717,7 +1724,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(1, 2))
         const int64_t beg = a.offsets[sid];
         const int L = int(a.offsets[sid + 1] - beg);
         const uint8_t* x = a.serve ? serve_fetch(a, beg, L, stage) : a.text + beg;
-        scan_string<CAP, MODE>(g, a, sid, x, L, bitsA, bitsB, listA, listB, msteps);
+        scan_string<CAP, M, kNarrow>(g, a, sid, x, L, bitsA, bitsB, listA, listB, msteps);
         if (a.serve) serve_finish(a, sid);
     }
     msteps[2] = msteps[3] - msteps[0] - msteps[1];

@@ -140,7 +140,8 @@ int Engine::scan_noalloc(const uint8_t* d_text, const int64_t* d_off, int32_t n,
             ADM_HIP_ERR(launch_plain(var_.plain_bs, n, plain_lds, st, dev_, a));
         } else {
             ADM_HIP_ERR(launch_order(d_off, n, ctr + kCtrBins, ctr + kCtrCursors, order, st));
-            ADM_HIP_ERR(launch_engine(var_.wpb, var_.cap, mode, false, grid, engine_lds, st, dev_, a));
+            ADM_HIP_ERR(
+                launch_engine(var_.wpb, var_.cap, engine_kmode(mode, false), false, grid, engine_lds, st, dev_, a));
         }
     }
     hipLaunchKernelGGL(noalloc_status_kernel, dim3(1), dim3(64), 0, st, ctr, d_status);

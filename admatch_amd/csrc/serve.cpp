@@ -327,7 +327,7 @@ int Engine::serve_ring(int32_t mode, const uint8_t* buf, const int64_t* offsets,
     // a ticket waits at most until the last arrival plus 10 s (100 MHz ticks)
     a.serve_wait_ticks = int64_t((arrival_s[n - 1] - arrival_s[0] + 10.0) * 1e8);
     if (hs) a.dense = c.d_dense;
-    const int32_t kmode = (hs && g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
+    const int32_t kmode = engine_kmode(mode, hs);
     ADM_RING_HIP(launch_fill(c.d_ctr, 0, kCtrWords, c.stream));
     ADM_RING_HIP(launch_engine(var_.wpb, var_.cap, kmode, hs, grid, lds, c.stream, dev_, a));
     out.waves = int32_t(waves);
