@@ -568,7 +568,39 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
 struct StepCtx {
     int c, c_prev, n_act, sparse, T, K, Lft, last_m;
     int pos, len;
+    // out-degree of the step's listed states on c: the share whose row is
+    // empty (-1 for a bitmap-only set) and the rows' entries; cycles of the
+    // step before
+    float deg_empty = -1.f, deg_entries = 0.f, last_cyc = 0.f;
 };
+
+// Out-degree bins of a step for the selector features: over the listed
+// states (always-active ones excluded: their rows are the implicit set),
+// the share with an empty row on byte c and the rows' total entries.  A
+// bitmap-only set gets -1 and the expected entries |T_c| * n / S.
+static inline __device__ void out_degree(const DevGraph& g, int c, const uint32_t* cl, int n_act, bool sparse,
+                                         StepCtx& ctx) {
+    if (!sparse || g.as_tab == nullptr) {
+        ctx.deg_empty = -1.f;
+        ctx.deg_entries = float(g.sym_off[c + 1] - g.sym_off[c]) * float(n_act) / float(g.S);
+        return;
+    }
+    int empty = 0, listed = 0, entries = 0;
+    for (int k0 = 0; k0 < n_act; k0 += 64) {
+        const int k = k0 + lane_id();
+        const uint32_t sv = k < n_act ? cl[k] : kTag;
+        const bool ok = !(sv & kTag);
+        const uint32_t e = ok ? g.as_tab[((sv & kIdMask) << 8) + uint32_t(c)] : 0u;
+        uint32_t len = e >> 24;
+        if (len == 255u) len = g.as_multi[e & kIdMask];
+        listed += ok ? 1 : 0;
+        empty += (ok && len == 0) ? 1 : 0;
+        entries += ok ? int(len) : 0;
+    }
+    listed = wave_sum(listed);
+    ctx.deg_empty = listed > 0 ? float(wave_sum(empty)) / float(listed) : 0.f;
+    ctx.deg_entries = float(wave_sum(entries));
+}
 
 __device__ __forceinline__ int heuristic_choice(const DevGraph& g, const StepCtx& s) {
     // AdMatch rule (src/AdMatch_kernel.cu:87) extended with an iNFA cost term
@@ -585,17 +617,17 @@ __device__ __forceinline__ float feature(const DevGraph& g, const StepCtx& s, in
         case 2: return __log2f(1.f + float(s.T));
         case 3: return float(s.K);
         case 4: return __log2f(1.f + float(s.Lft));
-        case 5: return __log2f(float(g.W));
+        case 5: return s.deg_empty;
         case 6: return word_char(uint32_t(s.c)) ? 1.f : 0.f;
         case 7: return float(s.c >> 5);
         case 8: return float(s.pos) / float(s.len > 0 ? s.len : 1);
-        case 9: return __log2f(1.f + float(s.len));
+        case 9: return __log2f(1.f + s.deg_entries);
         case 10: return float(s.sparse);
         case 11: return s.last_m == M_INFA ? 1.f : 0.f;
         case 12: return s.last_m == M_TKO ? 1.f : 0.f;
         case 13: return s.last_m == M_AS ? 1.f : 0.f;
         case 14: return __log2f(1.f + float(s.len - s.pos));
-        case 15: return __log2f(1.f + float(s.K) * float(g.W));
+        case 15: return __log2f(1.f + s.last_cyc);
         default: return 0.f;
     }
 }
@@ -642,9 +674,11 @@ constexpr int kPolicyWords = kPolicyBuckets * 2 * 16;
 static_assert(kPolicyWords == kPolicyTableWords, "policy table size");
 // Step context at which the network is evaluated for a table entry: the
 // bucket's middle count plus the mean always-active row, a step in the
-// middle of a kPolicyLen-byte string, and the method a step of that list
-// validity usually follows.
+// middle of a kPolicyLen-byte string, the method a step of that list
+// validity usually follows, half of the listed rows empty and one row entry
+// per active state, and a step before of kPolicyStepCycles cycles.
 constexpr int kPolicyLen = 1024;
+constexpr int kPolicyStepCycles = 1024;  // the step before, at a synthetic context
 
 __device__ __forceinline__ int policy_bucket(int n_act) { return min(kPolicyBuckets - 1, 31 - __clz(n_act + 1)); }
 
@@ -661,6 +695,9 @@ __device__ __forceinline__ StepCtx policy_ctx(const DevGraph& g, int c, int buck
     s.last_m = sparse ? M_AS : M_INFA;
     s.pos = kPolicyLen / 2;
     s.len = kPolicyLen;
+    s.deg_empty = sparse ? 0.5f : -1.f;
+    s.deg_entries = float(s.n_act);
+    s.last_cyc = float(kPolicyStepCycles);
     return s;
 }
 
@@ -1004,6 +1041,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
     ctx.last_m = M_INFA;
     ctx.c_prev = -1;
     int cached = -1;
+    uint64_t t_step = 0;  // clock at the last step's start (feature-evaluating modes)
     for (int b = 0; b < L; ++b) {
         // the next byte is read one step ahead (two-byte rows, word boundaries)
         const int c = c_next;
@@ -1040,6 +1078,18 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
         ctx.c = c;
         ctx.sparse = sparse;
         ctx.pos = b;
+        if constexpr (MODE == MODE_LOG || MODE == MODE_LEARNED_EVERY) {
+            // the modes that evaluate features: the step before's cycles, and
+            // the listed states' out-degree on this byte when the network or
+            // the log will read it
+            const uint64_t now = clock64();
+            ctx.last_cyc = b > 0 ? float(now - t_step) : 0.f;
+            t_step = now;
+            const bool need = MODE == MODE_LOG
+                                  ? ((uint64_t(sid) * 2654435761ull + uint64_t(b)) % uint64_t(a.log_stride)) == 0
+                                  : (cached < 0 || (b % a.mlp_every) == 0);
+            if (need) out_degree(g, c, cl, n_act, sparse, ctx);
+        }
         bool log_step = false;
         int m = choose_method<MODE>(g, a, sid, ctx, n_act, sparse, total_active, cached, log_step);
         // without two-byte rows the first positions of the implicit set are

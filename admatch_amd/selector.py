@@ -26,11 +26,16 @@ F, H, O = 16, 64, 3
 N_PARAMS = E.MLP_PARAMS
 WEIGHTS_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "selector_weights.npy")
 
+# The step features (device: feature() in engine_kernels.hpp): active-state
+# count, the byte's transition statistics, out-degree bins of the listed
+# states on the byte, the byte class, position and length, and the step
+# before (its method and its cycles).
 FEATURE_NAMES = [
-    "log2(1+active)", "active %", "log2(1+|T_c|)", "k_c", "log2(1+leftover_c)", "log2(W)", "word(c)",
-    "c/32", "pos/len", "log2(1+len)", "list valid", "last=iNFA", "last=TKO", "last=AS", "log2(1+remaining)",
-    "log2(1+k_c*W)",
+    "log2(1+active)", "active %", "log2(1+|T_c|)", "k_c", "log2(1+leftover_c)", "empty-row share (-1: bitmap)",
+    "word(c)", "c/32", "pos/len", "log2(1+row entries)", "list valid", "last=iNFA", "last=TKO", "last=AS",
+    "log2(1+remaining)", "log2(1+last step cycles)",
 ]
+POLICY_STEP_CYCLES = 1024  # the step before at a synthetic table context (kPolicyStepCycles)
 
 
 def _sig():
@@ -201,9 +206,10 @@ def policy_inputs(scratch: E.Scratch):
         for v in (0, 1):
             for c in range(256):
                 word = chr(c).isascii() and (chr(c).isalnum() or c == ord("_"))
-                rows.append([np.log2(1 + n), 100.0 * n / S, np.log2(1 + T[c]), K[c], np.log2(1 + Lft[c]), np.log2(W),
-                             float(word), float(c >> 5), 0.5, np.log2(1 + POLICY_LEN), float(v), float(v == 0), 0.0,
-                             float(v == 1), np.log2(1 + POLICY_LEN // 2), np.log2(1 + K[c] * W)])
+                rows.append([np.log2(1 + n), 100.0 * n / S, np.log2(1 + T[c]), K[c], np.log2(1 + Lft[c]),
+                             0.5 if v else -1.0, float(word), float(c >> 5), 0.5, np.log2(1 + n), float(v),
+                             float(v == 0), 0.0, float(v == 1), np.log2(1 + POLICY_LEN // 2),
+                             np.log2(1 + POLICY_STEP_CYCLES)])
                 allow.append(v == 1)
     return np.array(rows, dtype=np.float64), np.array(allow)
 
