@@ -106,10 +106,12 @@ enum SelectMode : int32_t {
     MODE_LEARNED_AS = 8,     // internal: policy table whose reachable list-valid entries all pick AS
 };
 
-// Hashed-storage kernels are also built with this bit added to the mode, for
-// rule sets without word-boundary transitions and with two-byte rows: the
-// boundary code and the path for missing rows are compiled out of the step
-// loop.
+// Kernels are also built with this bit added to the mode, for rule sets
+// without word-boundary transitions and with two-byte rows: the boundary code,
+// the path for missing rows and (LDS storage) the stream code are compiled out
+// of the step loop, which leaves the loop fewer scalar registers to spill.
+// Streams, the logging mode and the every-k-steps network mode use the full
+// kernels.
 constexpr int32_t kModeLean = 0x100;
 // LDS-storage kernels are also built with this bit added to the mode, for
 // state vectors of at most 64 * kTkoPassChunks words: a smaller TKO step and

@@ -479,7 +479,7 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     }
     int grid = 0;
     const bool hs = hashed(var_, mode) && !c.active_stream;  // streams use LDS storage
-    if (int rc = engine_grid(mode, n, hs, &grid, err)) return rc;
+    if (int rc = engine_grid(mode, n, hs, &grid, err, c.active_stream != nullptr)) return rc;
     if (hs) {
         if (c.dense_words < size_t(grid) * var_.wpb * 2 * size_t(g_.W)) {
             err = "dense bitmaps not prepared for this launch";
@@ -490,7 +490,7 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     if (prefix) ADM_HIP(launch_order(d_off, n, c.d_ctr + kCtrBins, c.d_ctr + kCtrCursors, c.d_order, st));
     if (!engine) return ADM_OK;
     const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
-    const int32_t kmode = engine_kmode(mode, hs);
+    const int32_t kmode = engine_kmode(mode, hs, c.active_stream != nullptr);
     ADM_HIP(hipEventRecord(c.ev_k0, st));
     ADM_HIP(launch_engine(var_.wpb, var_.cap, kmode, hs, grid, lds, st, dev_, a));
     ADM_HIP(hipEventRecord(c.ev_k1, st));
@@ -541,19 +541,22 @@ int Engine::prepare_dense(Context& c, int32_t mode, std::string& err) {
     return c.ensure_dense(size_t(grid) * var_.wpb * 2 * size_t(g_.W), err);
 }
 
-int32_t Engine::engine_kmode(int32_t mode, bool hs) const {
-    if (hs) return (g_.wb_count() == 0 && !g_.pair_off.empty()) ? (mode | kModeLean) : mode;
-    return g_.W <= 64 * kTkoPassChunks ? (mode | kModeNarrow) : mode;
+int32_t Engine::engine_kmode(int32_t mode, bool hs, bool stream) const {
+    const bool lean_rules = g_.wb_count() == 0 && !g_.pair_off.empty();
+    if (hs) return lean_rules ? (mode | kModeLean) : mode;
+    int32_t k = g_.W <= 64 * kTkoPassChunks ? (mode | kModeNarrow) : mode;
+    if (lean_rules && !stream && mode != MODE_LOG && mode != MODE_LEARNED_EVERY) k |= kModeLean;
+    return k;
 }
 
-int Engine::engine_grid(int32_t mode, int32_t n, bool hs, int* grid, std::string& err) {
+int Engine::engine_grid(int32_t mode, int32_t n, bool hs, int* grid, std::string& err, bool stream) {
     const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
     if (lds > max_lds_) {
         err = "rule set too large for the engine's LDS budget (lower the waves per workgroup)";
         return ADM_ERR_TOO_LARGE;
     }
     int blocks = 0;
-    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, engine_kmode(mode, hs), hs, lds, &blocks));
+    ADM_HIP(engine_occupancy(var_.wpb, var_.cap, engine_kmode(mode, hs, stream), hs, lds, &blocks));
     if (blocks <= 0) {
         err = "engine variant does not fit on a CU";
         return ADM_ERR_TOO_LARGE;

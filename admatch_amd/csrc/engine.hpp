@@ -236,11 +236,11 @@ class Engine {
     int64_t tko_distinct_rows() const { return tko_rows_; }  // after upload
 
    private:
-    int engine_grid(int32_t mode, int32_t n, bool hashed_storage, int* grid, std::string& err);
-    // the kernel instance of a mode: lean hashed kernels for rule sets
-    // without boundaries and with two-byte rows, narrow LDS kernels for
-    // small state vectors
-    int32_t engine_kmode(int32_t mode, bool hashed_storage) const;
+    int engine_grid(int32_t mode, int32_t n, bool hashed_storage, int* grid, std::string& err, bool stream = false);
+    // the kernel instance of a mode: lean kernels for rule sets without
+    // boundaries and with two-byte rows (not for streams), narrow LDS
+    // kernels for small state vectors
+    int32_t engine_kmode(int32_t mode, bool hashed_storage, bool stream = false) const;
     // phases of a launch sequence: the prefix is graph-capturable, the engine
     // kernel runs between ev_k0 and ev_k1
     enum LaunchPhase { kPhaseAll = 0, kPhasePrefix = 1, kPhaseEngine = 2 };

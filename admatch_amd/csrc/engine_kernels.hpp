@@ -19,6 +19,7 @@
 #pragma once
 
 #include "device.hpp"
+#include "graph.hpp"
 
 namespace admatch {
 
@@ -471,17 +472,26 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
     }
     // common case: one chunk of states, one of two-byte row entries, rows of
     // at most two destinations (a persistent state's row is itself plus the
-    // state it advances to): every lane's candidates in one pass
-    if (n_cur <= 64 && (!use_pair || pair.n <= 64) && !__ballot((e0 >> 24) >= 3)) {
+    // state it advances to, held in the entry): every lane's candidates in
+    // one pass
+    const uint32_t cnt0 = e0 >> 24;
+    const bool self0 = cnt0 == kAsSelfRow;
+    if (n_cur <= 64 && (!use_pair || pair.n <= 64) && !__ballot(cnt0 >= 3 && !self0)) {
         const bool okp = use_pair && lane_id() < pair.n;
-        const uint32_t cnt = e0 >> 24;
+        uint32_t cnt = cnt0;
         uint32_t x0 = e0 & kIdMask, x1 = 0;
-        const bool two = __ballot(cnt == 2) != 0;  // a row in the store
-        if (two && cnt == 2) {
-            const uint32_t o = x0;
-            x0 = g.as_multi[o];
-            x1 = g.as_multi[o + 1];
+        if (__ballot(cnt == 2)) {  // a row in the store
+            if (cnt == 2) {
+                const uint32_t o = x0;
+                x0 = g.as_multi[o];
+                x1 = g.as_multi[o + 1];
+            }
         }
+        if (self0) {
+            x1 = sv0 & kIdMask;
+            cnt = 2;
+        }
+        const bool two = __ballot(cnt == 2) != 0;
         if constexpr (SET::kBounded) {
             const bool n1 = fut.insert_if(okp, pair.v);
             const bool n2 = fut.insert_if(cnt != 0, x0);
@@ -524,11 +534,19 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
         }
         const uint32_t cnt = e >> 24;
         const uint32_t d = e & kIdMask;
-        const bool nw = cnt == 1 && fut.insert(d);
+        const bool self = cnt == kAsSelfRow;
+        const bool nw = (cnt == 1 || self) && fut.insert(d);
         n = list_append<CAP>(fl, n, nw, d);
         if (SET::kBounded && n > CAP) return n;
+        if (__ballot(self)) {
+            // the source state itself, the entry's second destination
+            const uint32_t own = (k0 == 0 ? sv0 : (self ? cl[k0 + lane_id()] : 0u)) & kIdMask;
+            const bool ns = self && fut.insert(own);
+            n = list_append<CAP>(fl, n, ns, own);
+            if (SET::kBounded && n > CAP) return n;
+        }
         // short rows: each lane walks its own row, two or four loads in flight
-        const int len = (cnt >= 2 && cnt < 255) ? int(cnt) : 0;
+        const int len = (cnt >= 2 && cnt < kAsSelfRow) ? int(cnt) : 0;
         if (__ballot(len > 0) && !__ballot(len > 2)) {
             // rows of two (a persistent state and the state it advances to)
             const uint32_t y0 = len ? g.as_multi[d] : 0u;
@@ -593,6 +611,7 @@ static inline __device__ void out_degree(const DevGraph& g, int c, const uint32_
         const uint32_t e = ok ? g.as_tab[((sv & kIdMask) << 8) + uint32_t(c)] : 0u;
         uint32_t len = e >> 24;
         if (len == 255u) len = g.as_multi[e & kIdMask];
+        if (len == kAsSelfRow) len = 2;
         listed += ok ? 1 : 0;
         empty += (ok && len == 0) ? 1 : 0;
         entries += ok ? int(len) : 0;
@@ -882,7 +901,7 @@ __device__ __forceinline__ void clear_bits(uint64_t* bits, const uint32_t* list,
     }
 }
 
-template <int CAP, int MODE, bool NARROW>
+template <int CAP, int MODE, bool NARROW, bool LEAN>
 __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const uint8_t* x, int L, uint64_t* bitsA,
                             uint64_t* bitsB, uint32_t* listA, uint32_t* listB, int* msteps) {
     const int W = g.W;
@@ -892,8 +911,9 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
     uint32_t* cl = listA;
     uint32_t* fl = listB;
     // streaming: the string continues a stream whose state and last byte
-    // class were saved by the previous chunk
-    const int slot = a.stream_ids != nullptr ? a.stream_ids[sid] : -1;
+    // class were saved by the previous chunk (LEAN: no streams, no boundary
+    // transitions, two-byte rows present)
+    const int slot = (!LEAN && a.stream_ids != nullptr) ? a.stream_ids[sid] : -1;
     const uint32_t sflags = slot >= 0 ? uint32_t(a.stream_flags[slot]) : 0u;
     const uint64_t* saved = slot >= 0 ? a.stream_state + size_t(slot) * W : nullptr;
     // invariant: both bitmaps are zero here.  The always-active states
@@ -906,8 +926,8 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
     bool sparse = n_act <= CAP;
     ByteWindow win{x, L, 0, 0u};
     if (L > 0) win.load(0);
-    const bool has_wb = g.wb_count > 0;
-    const bool have_pair = g.pair_off != nullptr;
+    const bool has_wb = !LEAN && g.wb_count > 0;
+    const bool have_pair = LEAN || g.pair_off != nullptr;
     // States active but held implicitly (in neither the bitmap nor the list):
     // imp >= 0: the always-active row of byte imp (the always-active states
     // and the first positions they reached on that byte); imp == -2: the
@@ -1745,8 +1765,9 @@ __global__ __launch_bounds__(64 * WPB) ADM_HASHED_ATTR void engine_h_kernel(DevG
 template <int WPB, int CAP, int MODE>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu((MODE & kModeNarrow) ? 4 : 1, (MODE & kModeNarrow) ? 4 : 2))) void
 engine_kernel(DevGraph g, ScanArgs a) {
-    constexpr int M = MODE & ~kModeNarrow;
+    constexpr int M = MODE & ~(kModeNarrow | kModeLean);
     constexpr bool kNarrow = (MODE & kModeNarrow) != 0;
+    constexpr bool kLean = (MODE & kModeLean) != 0;
     // the per-byte statistics stay in device memory (scalar loads): a 4 KB
     // copy per workgroup in LDS would cost large rule sets a workgroup per CU
     extern __shared__ uint64_t lds[];
@@ -1774,7 +1795,7 @@ engine_kernel(DevGraph g, ScanArgs a) {
         const int64_t beg = a.offsets[sid];
         const int L = int(a.offsets[sid + 1] - beg);
         const uint8_t* x = a.serve ? serve_fetch(a, beg, L, stage) : a.text + beg;
-        scan_string<CAP, M, kNarrow>(g, a, sid, x, L, bitsA, bitsB, listA, listB, msteps);
+        scan_string<CAP, M, kNarrow, kLean>(g, a, sid, x, L, bitsA, bitsB, listA, listB, msteps);
         if (a.serve) serve_finish(a, sid);
     }
     msteps[2] = msteps[3] - msteps[0] - msteps[1];

@@ -372,12 +372,16 @@ void build_as(Graph& g) {
                 ent = (1u << 24) | row[0];
                 continue;
             }
+            if (row.size() == 2 && (row[0] == uint32_t(s) || row[1] == uint32_t(s))) {
+                ent = (kAsSelfRow << 24) | (row[0] == uint32_t(s) ? row[1] : row[0]);
+                continue;
+            }
             const size_t idx = g.as_multi.size();
             if (idx + row.size() + 1 >= (size_t(1) << 24)) {
                 g.as_ok = false;
                 break;
             }
-            if (row.size() >= 255) {
+            if (row.size() >= kAsSelfRow) {
                 g.as_multi.push_back(uint32_t(row.size()));
                 ent = (255u << 24) | uint32_t(idx);
             } else {

@@ -26,6 +26,10 @@ constexpr int kWordBoundary = 256;  // src/common.h:15
 constexpr int kMaxK = 1024;        // src/common.h:21
 constexpr size_t kMaxPairEntries = size_t(1) << 24;
 constexpr int32_t kMaxStates = 1 << 24;  // ids fit the 24-bit payload of the AS table
+// AS table count of a row of two destinations one of which is the source
+// state itself: the other one is the entry's payload, so the row needs no
+// load from as_multi (a persistent state that advances on the byte).
+constexpr uint32_t kAsSelfRow = 254;
 
 struct Edge {
     uint32_t src, dst;
@@ -63,8 +67,9 @@ struct Graph {
     // Device form of the AS rows, one 32-bit entry per (state, byte), state
     // major (as_tab[s*256 + c], so the 256 entries of an active state share
     // one kilobyte of cache lines): count in the top 8 bits (0 = empty, 255 =
-    // long row whose count is as_multi[idx]), then either the single
-    // destination (count 1) or the row's index into as_multi.  Rows of
+    // long row whose count is as_multi[idx], kAsSelfRow = the source state
+    // plus the payload), then either the single destination (count 1 and
+    // kAsSelfRow) or the row's index into as_multi.  Rows of
     // persistent states include the state itself, so the AS step needs no
     // separate persistent mask.  Always-active destinations are left out:
     // the AS step holds those states implicitly.

@@ -254,6 +254,8 @@ def _decode_as_tab(s, S):
             cnt, pay = e >> 24, e & 0xFFFFFF
             if cnt == 1:
                 rows[(c, int(st))] = [pay]
+            elif cnt == 254:  # the state itself plus the payload
+                rows[(c, int(st))] = sorted([int(st), pay])
             elif cnt == 255:
                 rows[(c, int(st))] = multi[pay + 1: pay + 1 + multi[pay]].tolist()
             else:
