@@ -1697,6 +1697,31 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, con
     wave_sync();
 }
 
+// The LDS-storage engine kernel's (DevGraph, ScanArgs) arguments, read
+// through the kernarg segment pointer made opaque once per string.  Reading
+// the by-value parameters directly makes every field the kernel uses a load at
+// kernel entry, live in scalar registers for the whole kernel, and the step
+// loop then spills dozens of them to vector lanes and reloads them every step;
+// through the opaque pointer a field is loaded where it is used, at most once
+// per string.  The explicit arguments start at offset 0 of the segment, each
+// at its own alignment (the HSA kernel argument layout).  The hashed kernel
+// reads its parameters directly: it has fewer spills to begin with and ran
+// 3% slower through the pointer on the sparse large set.
+struct KernelArgs {
+    const DevGraph* g;
+    const ScanArgs* a;
+};
+__device__ __forceinline__ KernelArgs kernel_args() {
+    using KG = const __attribute__((address_space(4))) DevGraph;
+    using KA = const __attribute__((address_space(4))) ScanArgs;
+    using KC = const __attribute__((address_space(4))) char;
+    constexpr size_t kArgsOff = (sizeof(DevGraph) + alignof(ScanArgs) - 1) / alignof(ScanArgs) * alignof(ScanArgs);
+    KG* g = (KG*)__builtin_amdgcn_kernarg_segment_ptr();
+    KA* a = (KA*)((KC*)__builtin_amdgcn_kernarg_segment_ptr() + kArgsOff);
+    asm volatile("" : "+s"(g), "+s"(a));
+    return KernelArgs{(const DevGraph*)g, (const ScanArgs*)a};
+}
+
 template <int CAP>
 struct LaneH {
     // u64 words per wave: two lists of CAP + 64 u32 and two tables of
@@ -1781,27 +1806,30 @@ engine_kernel(DevGraph g, ScanArgs a) {
     for (int j = lane_id(); j < 2 * g.W; j += 64) base[j] = 0ull;
     wave_sync();
     int msteps[4] = {0, 0, 0, 0};  // iNFA, TKO, AS (filled below), bytes scanned
-    uint8_t* stage = a.serve_stage + (size_t(blockIdx.x) * WPB + wave) * size_t(a.serve_slot);
     for (;;) {
+        const KernelArgs k = kernel_args();
+        const ScanArgs& ka = *k.a;
         int idx = 0;
-        if (lane_id() == 0) idx = atomicAdd(a.queue, 1);
+        if (lane_id() == 0) idx = atomicAdd(ka.queue, 1);
         idx = uniform(__builtin_amdgcn_readlane(idx, 0));
-        if (a.serve) {
-            if (!serve_wait(a, idx)) break;
-        } else if (idx >= a.n) {
+        if (ka.serve) {
+            if (!serve_wait(ka, idx)) break;
+        } else if (idx >= ka.n) {
             break;
         }
-        const int sid = a.order ? a.order[idx] : idx;
-        const int64_t beg = a.offsets[sid];
-        const int L = int(a.offsets[sid + 1] - beg);
-        const uint8_t* x = a.serve ? serve_fetch(a, beg, L, stage) : a.text + beg;
-        scan_string<CAP, M, kNarrow, kLean>(g, a, sid, x, L, bitsA, bitsB, listA, listB, msteps);
-        if (a.serve) serve_finish(a, sid);
+        const int sid = ka.order ? ka.order[idx] : idx;
+        const int64_t beg = ka.offsets[sid];
+        const int L = int(ka.offsets[sid + 1] - beg);
+        uint8_t* stage = ka.serve_stage + (size_t(blockIdx.x) * WPB + wave) * size_t(ka.serve_slot);
+        const uint8_t* x = ka.serve ? serve_fetch(ka, beg, L, stage) : ka.text + beg;
+        scan_string<CAP, M, kNarrow, kLean>(*k.g, *k.a, sid, x, L, bitsA, bitsB, listA, listB, msteps);
+        if (ka.serve) serve_finish(ka, sid);
     }
     msteps[2] = msteps[3] - msteps[0] - msteps[1];
-    if (a.method_steps != nullptr && lane_id() == 0)
+    const ScanArgs& ka = *kernel_args().a;
+    if (ka.method_steps != nullptr && lane_id() == 0)
         for (int m = 0; m < 3; ++m)
-            if (msteps[m]) atomicAdd(&a.method_steps[m], (unsigned long long)msteps[m]);
+            if (msteps[m]) atomicAdd(&ka.method_steps[m], (unsigned long long)msteps[m]);
 }
 
 }  // namespace admatch
