@@ -416,7 +416,7 @@ __device__ __forceinline__ void tko_groups(const DevGraph& g, const TkoRec& tk, 
 // len_l destinations starting at as_multi[start_l]; each iteration every
 // lane fetches one destination (coalesced within a row).
 template <int CAP, typename SET>
-__device__ int multi_rows(const DevGraph& g, uint32_t start, int len, const SET& fut, uint32_t* fl, int n) {
+__device__ int multi_rows(const uint32_t* as_multi, uint32_t start, int len, const SET& fut, uint32_t* fl, int n) {
     const int incl = wave_scan(len);
     const int excl = incl - len;
     const int tot = uniform(__shfl(incl, 63, 64));
@@ -433,13 +433,21 @@ __device__ int multi_rows(const DevGraph& g, uint32_t start, int len, const SET&
         const uint32_t so = uint32_t(__shfl(int(start), lo, 64));
         const int exo = __shfl(excl, lo, 64);
         const bool ok = t < tot;
-        const uint32_t d = ok ? g.as_multi[so + uint32_t(t - exo)] : 0u;
+        const uint32_t d = ok ? as_multi[so + uint32_t(t - exo)] : 0u;
         const bool nw = ok && fut.insert(d);
         n = list_append<CAP>(fl, n, nw, d);
         if (SET::kBounded && n > CAP) return n;
     }
     return n;
 }
+
+// The tables an AS step reads, loaded once per string by the caller (kept in
+// scalar registers across the step loop rather than reloaded per step).
+struct AsRows {
+    const uint32_t* tab;       // DevGraph::as_tab
+    const uint32_t* multi;     // DevGraph::as_multi
+    const uint32_t* pair_dst;  // DevGraph::pair_dst
+};
 
 // Prefetched slice of a precomputed row list: the first 64 entries, one per
 // lane, plus where the list lives.
@@ -458,10 +466,10 @@ struct RowSlice {
 // neither gathered nor stored.  clr (or null): the bitmap of the listed
 // states, whose words are cleared as the list is read.
 template <int CAP, typename SET>
-__device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_pair, const uint32_t* cl, int n_cur,
+__device__ int as_step(const AsRows& g, int c, const RowSlice& pair, bool use_pair, const uint32_t* cl, int n_cur,
                        const SET& fut, uint32_t* fl, uint64_t* clr) {
     // state-major table: entry (s, c) at s*256 + c
-    const uint32_t* tab = g.as_tab + c;
+    const uint32_t* tab = g.tab + c;
     // the first chunk's table gathers are in flight while the two-byte rows
     // are merged
     const uint32_t sv0 = lane_id() < n_cur ? cl[lane_id()] : kTag;
@@ -483,8 +491,8 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
         if (__ballot(cnt == 2)) {  // a row in the store
             if (cnt == 2) {
                 const uint32_t o = x0;
-                x0 = g.as_multi[o];
-                x1 = g.as_multi[o + 1];
+                x0 = g.multi[o];
+                x1 = g.multi[o + 1];
             }
         }
         if (self0) {
@@ -549,8 +557,8 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
         const int len = (cnt >= 2 && cnt < kAsSelfRow) ? int(cnt) : 0;
         if (__ballot(len > 0) && !__ballot(len > 2)) {
             // rows of two (a persistent state and the state it advances to)
-            const uint32_t y0 = len ? g.as_multi[d] : 0u;
-            const uint32_t y1 = len ? g.as_multi[d + 1] : 0u;
+            const uint32_t y0 = len ? g.multi[d] : 0u;
+            const uint32_t y1 = len ? g.multi[d + 1] : 0u;
             const bool n0 = len && fut.insert(y0);
             n = list_append<CAP>(fl, n, n0, y0);
             if (SET::kBounded && n > CAP) return n;
@@ -560,7 +568,7 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
         } else if (__ballot(len > 0)) {
             uint32_t x[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) x[j] = j < len ? g.as_multi[d + j] : 0u;
+            for (int j = 0; j < 4; ++j) x[j] = j < len ? g.multi[d + j] : 0u;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const bool nj = j < len && fut.insert(x[j]);
@@ -568,7 +576,7 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
                 if (SET::kBounded && n > CAP) return n;
             }
             for (int j = 4; __ballot(j < len); ++j) {
-                const uint32_t y = j < len ? g.as_multi[d + j] : 0u;
+                const uint32_t y = j < len ? g.multi[d + j] : 0u;
                 const bool nj = j < len && fut.insert(y);
                 n = list_append<CAP>(fl, n, nj, y);
                 if (SET::kBounded && n > CAP) return n;
@@ -576,7 +584,7 @@ __device__ int as_step(const DevGraph& g, int c, const RowSlice& pair, bool use_
         }
         // long rows (255 or more destinations): flattened over the wave
         const bool lng = cnt == 255;
-        if (__ballot(lng)) n = multi_rows<CAP>(g, d + 1, lng ? int(g.as_multi[d]) : 0, fut, fl, n);
+        if (__ballot(lng)) n = multi_rows<CAP>(g.multi, d + 1, lng ? int(g.multi[d]) : 0, fut, fl, n);
         if (SET::kBounded && n > CAP) return n;
     }
     return n;
@@ -1036,15 +1044,18 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
     // start, a non-word position) against it
     if (has_wb && L > 0 && word_char(uint32_t(c_next)) != ((sflags & 2u) != 0)) wb_pass();
 
+    // the tables of the AS steps, loaded once per string
+    const uint32_t* const pair_off = g.pair_off;
+    const AsRows rows{g.as_tab, g.as_multi, g.pair_dst};
     // two-byte rows of (byte, next byte), first 64 entries loaded a step ahead
     auto load_pair = [&](int c1, int c2) -> RowSlice {
         RowSlice r;
         // read-only offsets: scalar loads
         using CU32 = const __attribute__((address_space(4))) uint32_t;
-        const CU32* o = (CU32*)(g.pair_off) + (c1 * 256 + c2);
+        const CU32* o = (CU32*)(pair_off) + (c1 * 256 + c2);
         r.b = int(o[0]);
         r.n = int(o[1]) - r.b;
-        r.v = lane_id() < r.n ? g.pair_dst[r.b + lane_id()] : 0u;
+        r.v = lane_id() < r.n ? rows.pair_dst[r.b + lane_id()] : 0u;
         return r;
     };
     RowSlice pair{0u, 0, 0};
@@ -1121,7 +1132,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
         int next_imp = -1;
         if (!log_step) {
             if (m == M_AS) {
-                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl, cur);
+                n_fut = as_step<CAP>(rows, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl, cur);
                 fsp = n_fut <= CAP;
                 next_imp = c;
             } else {
@@ -1136,7 +1147,7 @@ __device__ void scan_string(const DevGraph& g, const ScanArgs& a, int sid, const
             if (sparse && can_as) {
                 __builtin_amdgcn_s_waitcnt(0);
                 const uint64_t t0 = clock64();
-                n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl, nullptr);
+                n_fut = as_step<CAP>(rows, c, pair, have_pair && imp >= 0, cl, n_act, BitSet{fut}, fl, nullptr);
                 fsp = n_fut <= CAP;
                 wave_sync();
                 __builtin_amdgcn_s_waitcnt(0);
@@ -1555,7 +1566,8 @@ __device__ void scan_string_h(const DevGraph& g, const ScanArgs& a, int sid, con
         int n_fut = 0;
         bool fsp = false;
         if (m == M_AS) {
-            n_fut = as_step<CAP>(g, c, pair, have_pair && imp >= 0, cl, n_act, hf, fl, nullptr);
+            n_fut = as_step<CAP>(AsRows{g.as_tab, g.as_multi, g.pair_dst}, c, pair, have_pair && imp >= 0, cl, n_act,
+                                 hf, fl, nullptr);
             wave_sync();
             if (n_fut <= CAP) {
                 fsp = true;
