@@ -414,6 +414,20 @@ int Engine::upload(std::string& err) {
     dev_.wb_begin = g_.sym_off[kWordBoundary];
     dev_.wb_count = g_.wb_count();
     dev_.wb_parallel = g_.wb_chain_free() ? 1 : 0;
+    {
+        // the LDS engine kernel reads its arguments through the kernarg
+        // segment pointer: check once that they sit where it looks
+        int32_t* d_probe = nullptr;
+        ADM_HIP(hipMalloc(&d_probe, sizeof(int32_t)));
+        dev_allocs_.push_back(d_probe);
+        int32_t h_probe = 0;
+        ADM_HIP(launch_kernarg_probe(dev_, d_probe, nullptr));
+        ADM_HIP(hipMemcpy(&h_probe, d_probe, sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (h_probe != 1) {
+            err = "engine kernel arguments are not where the kernel reads them (kernel argument layout)";
+            return ADM_ERR_HIP;
+        }
+    }
     dev_ready_ = true;
     return ADM_OK;
 }

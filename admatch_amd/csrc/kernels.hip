@@ -182,6 +182,26 @@ hipError_t launch_plain(int bs, int n, size_t lds, hipStream_t st, const DevGrap
         default: return hipErrorInvalidValue;
     }
 }
+// The engine kernel's (DevGraph, ScanArgs) signature: kernel_args() against
+// the parameters, once per rule-set upload.
+__global__ void kernarg_probe_kernel(DevGraph g, ScanArgs a) {
+    if (threadIdx.x == 0) {
+        const KernelArgs k = kernel_args();
+        const bool same = k.g->S == g.S && k.g->W == g.W && k.g->as_tab == g.as_tab && k.a->n == a.n &&
+                          k.a->overflow == a.overflow && k.a->log_stride == a.log_stride;
+        *a.overflow = same ? 1 : 2;
+    }
+}
+
+hipError_t launch_kernarg_probe(const DevGraph& g, int32_t* out, hipStream_t st) {
+    ScanArgs a{};
+    a.n = 0x5a5a;
+    a.log_stride = 0x3c3c;
+    a.overflow = out;
+    hipLaunchKernelGGL(kernarg_probe_kernel, dim3(1), dim3(64), 0, st, g, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_policy(const DevGraph& g, const float* mlp, uint32_t* table, hipStream_t st) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, st, reinterpret_cast<int32_t*>(table), 0,
                        int64_t(kPolicyWords));
