@@ -110,14 +110,27 @@ __global__ void stream_commit_kernel(const int32_t* ids, int32_t W, const uint64
 
 // counters layout (int32): [0] queue, [1] overflow, [2..3] pool_top (u64),
 // [4..4+kBins) bin counts, [4+kBins..4+2kBins) bin cursors
+// Strings fall into a handful of length bins, so each workgroup counts in LDS
+// and adds one total per bin to the global counters (a global atomic per
+// string would queue tens of thousands of them on a few addresses).
 __global__ void bin_count_kernel(const int64_t* offsets, int32_t n, int32_t* bins) {
+    __shared__ int32_t cnt[kBins];
+    for (int t = threadIdx.x; t < kBins; t += blockDim.x) cnt[t] = 0;
+    __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) atomicAdd(&bins[len_bin(offsets[i + 1] - offsets[i])], 1);
+    if (i < n) atomicAdd(&cnt[len_bin(offsets[i + 1] - offsets[i])], 1);
+    __syncthreads();
+    for (int t = threadIdx.x; t < kBins; t += blockDim.x)
+        if (cnt[t]) atomicAdd(&bins[t], cnt[t]);
 }
 
+// Each workgroup ranks its strings within their bins in LDS, reserves one
+// range per bin with a global atomic, and writes its strings there.
 __global__ void bin_scatter_kernel(const int64_t* offsets, int32_t n, const int32_t* bins, int32_t* cursors,
                                    int32_t* order) {
     __shared__ int32_t start[kBins];
+    __shared__ int32_t cnt[kBins];
+    __shared__ int32_t base[kBins];
     if (threadIdx.x == 0) {
         int32_t s = 0;
         for (int b = 0; b < kBins; ++b) {
@@ -125,12 +138,19 @@ __global__ void bin_scatter_kernel(const int64_t* offsets, int32_t n, const int3
             s += bins[b];
         }
     }
+    for (int t = threadIdx.x; t < kBins; t += blockDim.x) cnt[t] = 0;
     __syncthreads();
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = 0, r = 0;
     if (i < n) {
-        const int b = len_bin(offsets[i + 1] - offsets[i]);
-        order[start[b] + atomicAdd(&cursors[b], 1)] = i;
+        b = len_bin(offsets[i + 1] - offsets[i]);
+        r = atomicAdd(&cnt[b], 1);
     }
+    __syncthreads();
+    for (int t = threadIdx.x; t < kBins; t += blockDim.x)
+        if (cnt[t]) base[t] = atomicAdd(&cursors[t], cnt[t]);
+    __syncthreads();
+    if (i < n) order[start[b] + base[b] + r] = i;
 }
 
 
