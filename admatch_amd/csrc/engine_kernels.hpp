@@ -556,7 +556,7 @@ __device__ int as_step(const AsRows& g, int c, const RowSlice& pair, bool use_pa
         // short rows: each lane walks its own row, two or four loads in flight
         const int len = (cnt >= 2 && cnt < kAsSelfRow) ? int(cnt) : 0;
         if (__ballot(len > 0) && !__ballot(len > 2)) {
-            // rows of two (a persistent state and the state it advances to)
+            // rows of two in the store (neither entry the state itself)
             const uint32_t y0 = len ? g.multi[d] : 0u;
             const uint32_t y1 = len ? g.multi[d + 1] : 0u;
             const bool n0 = len && fut.insert(y0);
@@ -582,7 +582,8 @@ __device__ int as_step(const AsRows& g, int c, const RowSlice& pair, bool use_pa
                 if (SET::kBounded && n > CAP) return n;
             }
         }
-        // long rows (255 or more destinations): flattened over the wave
+        // long rows (kAsSelfRow or more destinations, count in the store):
+        // flattened over the wave
         const bool lng = cnt == 255;
         if (__ballot(lng)) n = multi_rows<CAP>(g.multi, d + 1, lng ? int(g.multi[d]) : 0, fut, fl, n);
         if (SET::kBounded && n > CAP) return n;
