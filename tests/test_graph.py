@@ -263,6 +263,30 @@ def _decode_as_tab(s, S):
     return rows
 
 
+def test_device_as_table_row_lengths(tmp_path):
+    # the entry forms at their boundaries: one destination inline, rows of 2
+    # and 253 in the store, 254 in the long form (count stored first), and a
+    # persistent state's row of itself plus one destination in the entry
+    lines = ["600"]
+    lines += [f"0 -> {d} : 97" for d in range(1, 255)]     # 254 destinations
+    lines += [f"0 -> {d} : 98" for d in range(1, 254)]     # 253
+    lines += ["0 -> 1 : 99", "0 -> 2 : 99", "0 -> 5 : 100"]  # 2 and 1
+    lines += ["300 -> 300 : 0|255", "300 -> 301 : 120", "300 -> 302 : 121", "300 -> 303 : 121"]
+    lines += ["400 -> 300 : 101", "0 : initial", "400 : initial", "301 : accepting 1"]
+    p = tmp_path / "rows.nfa"
+    p.write_text("\n".join(lines) + "\n")
+    s, o = same_as_oracle(str(p))
+    tab = s.table(10).reshape(o.S, 256)
+    cnt = tab >> 24
+    assert (cnt[0, 97], cnt[0, 98], cnt[0, 99], cnt[0, 100]) == (255, 253, 2, 1)
+    assert cnt[300, 120] == 254 and (tab[300, 120] & 0xFFFFFF) == 301  # {300, 301}
+    assert cnt[300, 121] == 3 and cnt[300, 0] == 1 and (tab[300, 0] & 0xFFFFFF) == 300
+    rows = _decode_as_tab(s, o.S)
+    assert rows[(97, 0)] == list(range(1, 255)) and rows[(98, 0)] == list(range(1, 254))
+    assert rows[(120, 300)] == [300, 301] and rows[(121, 300)] == [300, 302, 303]
+    s.close()
+
+
 @pytest.mark.parametrize("name", ["crs_g6.nfa.gz", "synth_small.nfa.gz"])
 def test_device_as_table(gold_dir, name):
     # device rows = AS rows plus the state itself for persistent states;
