@@ -28,6 +28,23 @@ def make_case(tmp_path, kind, seed):
     if kind == "random":
         text = random_nfa(seed, S=200, n_edges=900)
         strs = [bytes(rng.choice(list(b"ab_ Z\x00"), size=int(rng.integers(0, 300)))) for _ in range(150)]
+    elif kind == "rows":
+        # AS table entries at their length boundaries (one destination, 2 and
+        # 253 in the store, 254 in the long form, a persistent state plus one)
+        # and an always-active state, so the two-byte rows exist
+        lines = ["600"]
+        lines += [f"0 -> {d} : 97" for d in range(1, 255)]
+        lines += [f"0 -> {d} : 98" for d in range(1, 254)]
+        lines += ["0 -> 1 : 99", "0 -> 2 : 99", "0 -> 5 : 100"]
+        lines += [f"{d} -> {d + 300} : 120" for d in range(1, 255, 7)]
+        lines += ["300 -> 300 : 0|255", "300 -> 301 : 120", "300 -> 302 : 121", "300 -> 303 : 121"]
+        lines += ["500 -> 500 : 0|255", "500 -> 0 : 122", "500 -> 300 : 119", "0 : initial", "500 : initial"]
+        lines += ["300 : accepting 1", "303 : accepting 2", "5 : accepting 3", "254 : accepting 4", "553 : accepting 5"]
+        text = "\n".join(lines) + "\n"
+        # random prefixes, and endings that leave accepting states active
+        ends = [b"", b"wx", b"zd", b"za", b"zax", b"zbx", b"zb", b"wyy", b"zc"]
+        strs = [bytes(rng.choice(list(b"abcdxyzw "), size=int(rng.integers(0, 200)))) + ends[int(rng.integers(0, 9))]
+                for _ in range(150)]
     elif kind == "shift":
         text = shift_nfa(seed)
         strs = [bytes(rng.choice(list(b"abcx"), size=int(rng.integers(0, 300)))) for _ in range(150)]
@@ -48,7 +65,8 @@ def first_diff(got, exp):
     return None
 
 
-@pytest.mark.parametrize("kind,seed", [("random", 0), ("random", 1), ("shift", 0), ("burst", 0), ("burst", 1)])
+@pytest.mark.parametrize("kind,seed", [("random", 0), ("random", 1), ("shift", 0), ("burst", 0), ("burst", 1),
+                                       ("rows", 0)])
 def test_storages_vs_oracle(tmp_path, kind, seed):
     path, strs, exp = make_case(tmp_path, kind, seed)
     buf, off = E.pack_strings(strs)
