@@ -217,8 +217,12 @@ int ita_info(ita_handle* h, int64_t* out11);
  * (int32), 4 TKO lim (u64, K*W), 5 TKO leftover offsets (258 int32),
  * 6 TKO leftover (2*n uint32), 7 AS offsets (S*256+1 uint32), 8 AS dst
  * (uint32), 9 init/persistent/accept vectors (3*W u64), 10 device AS
- * table (S*256 uint32, entry of state s on byte c at s*256 + c), 11 its
- * multi-row store (uint32), 12 always-active row offsets (257 int32),
+ * table (S*256 uint32, entry of state s on byte c at s*256 + c: row length
+ * in the top 8 bits, 1 = the low 24 bits are the destination, 254 = the
+ * row is s itself and the low 24 bits, 255 = long row whose length is the
+ * store word at the low 24 bits, followed by the row, otherwise the row
+ * starts at that store index), 11 its multi-row store (uint32),
+ * 12 always-active row offsets (257 int32),
  * 13 always-active rows (uint32), 14 two-byte row offsets (65537 uint32),
  * 15 two-byte rows (uint32), 16 the learned selector's policy table
  * (512 uint32, after ita_set_selector: method of (bucket b, list valid v,
