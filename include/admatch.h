@@ -68,7 +68,9 @@ typedef struct ita_handle ita_handle;
  * calls exit(-1), Interface.cu:28-31). */
 int ita_create(uint16_t flags, const char* nfa_path, ita_handle** out);
 
-/* Upload the tables to the GPU (reference allocScratch, Interface.cu:42-116). */
+/* Upload the tables to the GPU (reference allocScratch, Interface.cu:42-116).
+ * It also runs a one-wave check that the engine kernel finds its arguments
+ * where it reads them; ITA_ERR_HIP if not (or on any HIP error). */
 int ita_alloc(ita_handle* h);
 
 /* Release device memory (reference freeScratch, Interface.cu:124-142). */
