@@ -277,6 +277,15 @@ int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3) {
     return ITA_OK;
 }
 
+int ita_last_kernel(ita_handle* h, int32_t* out2) {
+    if (int rc = check(h)) return rc;
+    if (!out2) return fail(ITA_ERR_ARG, "null output");
+    const auto& st = h->eng.stats();
+    out2[0] = st.kernel_mode;
+    out2[1] = st.kernel_store;
+    return ITA_OK;
+}
+
 int ita_last_times(ita_handle* h, float* out6) {
     if (int rc = check(h)) return rc;
     if (!out6) return fail(ITA_ERR_ARG, "null output");

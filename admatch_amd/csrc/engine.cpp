@@ -484,6 +484,8 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
             return ADM_ERR_TOO_LARGE;
         }
         a.order = nullptr;
+        c.stats.kernel_mode = -1;
+        c.stats.kernel_store = 0;
         if (prefix) ADM_HIP(launch_plain(var_.plain_bs, n, lds, st, dev_, a));
         return ADM_OK;
     }
@@ -505,6 +507,8 @@ int Engine::launch(Context& c, const uint8_t* d_text, const int64_t* d_off, int3
     if (!engine) return ADM_OK;
     const size_t lds = hs ? engine_lds_bytes(var_, mode) : engine_lds_bytes(Variant{var_.wpb, var_.cap, var_.plain_bs, 0});
     const int32_t kmode = engine_kmode(mode, hs, c.active_stream != nullptr);
+    c.stats.kernel_mode = kmode;
+    c.stats.kernel_store = hs ? 1 : 0;
     ADM_HIP(hipEventRecord(c.ev_k0, st));
     ADM_HIP(launch_engine(var_.wpb, var_.cap, kmode, hs, grid, lds, st, dev_, a));
     ADM_HIP(hipEventRecord(c.ev_k1, st));

@@ -53,6 +53,10 @@ struct ScanStats {
     float alloc_ms = 0.f, h2d_ms = 0.f, d2h_ms = 0.f, map_ms = 0.f;
     unsigned long long method_steps[3] = {0, 0, 0};
     long long pool_used = 0;
+    // engine kernel instance of the last launch: its mode argument (the mode
+    // plus the kModeLean / kModeNarrow bits), -1 for the plain kernel; and
+    // its storage (1: hashed lists)
+    int32_t kernel_mode = -1, kernel_store = 0;
 };
 
 // Everything one in-flight scan needs besides the rule set: a stream,

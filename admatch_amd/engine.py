@@ -31,6 +31,8 @@ HEURISTIC_SELECTOR = 1 << 7
 
 MODE_INFA, MODE_TKO, MODE_AS, MODE_ADMATCH, MODE_HEURISTIC, MODE_LEARNED, MODE_LOG = range(7)
 KIND_PLAIN, KIND_ENGINE = 0, 1
+# bits of the engine kernel's mode argument (last_kernel): lean and narrow instances
+LEAN_BIT, NARROW_BIT = 0x100, 0x200
 MODE_NAMES = {MODE_INFA: "infa", MODE_TKO: "tko", MODE_AS: "as", MODE_ADMATCH: "admatch",
               MODE_HEURISTIC: "heuristic", MODE_LEARNED: "learned", MODE_LOG: "log"}
 
@@ -44,7 +46,7 @@ EXPORTED = [
     "ita_fetch_log", "ita_info", "ita_table", "ita_last_error", "ita_selector_train", "ita_selector_grad",
     "ita_selector_forward", "ita_serve_trace", "ita_stream_create", "ita_stream_reset", "ita_stream_scan",
     "ita_stream_destroy", "ita_last_results", "ita_scan_device_noalloc", "ita_set_policy", "ita_serve_ring",
-    "ita_last_times", "ita_set_tko_threshold",
+    "ita_last_times", "ita_set_tko_threshold", "ita_last_kernel",
 ]
 
 _LIB = None
@@ -75,6 +77,7 @@ def lib():
             "ita_scan_device_noalloc": (i32, [vp, vp, vp, i32, i32, i32, vp, vp, vp, vp, i64, vp]),
             "ita_last_stats": (i32, [vp, vp, vp]),
             "ita_last_times": (i32, [vp, vp]),
+            "ita_last_kernel": (i32, [vp, vp]),
             "ita_set_variant": (i32, [vp, i32, i32, i32, i32]),
             "ita_get_variant": (i32, [vp, vp, vp, vp, vp]),
             "ita_set_tko_threshold": (i32, [vp, i32]),
@@ -219,6 +222,13 @@ class Scratch:
         steps = np.zeros(3, dtype=np.uint64)
         _err(lib().ita_last_stats(self._h, ctypes.byref(ms), _p(steps)))
         return ms.value, steps.tolist()
+
+    def last_kernel(self):
+        """(mode argument, storage) of the last scan's kernel: the mode plus
+        LEAN_BIT / NARROW_BIT, -1 for the plain kernel; storage 1 = hashed."""
+        t = np.zeros(2, dtype=np.int32)
+        _err(lib().ita_last_kernel(self._h, _p(t)))
+        return int(t[0]), int(t[1])
 
     def last_times(self) -> dict:
         """ms of the last scan: launch sequence, engine kernel alone, and the

@@ -155,6 +155,14 @@ int64_t ita_last_results(ita_handle* h, int32_t* out_ids, int64_t cap);
  * (iNFA, TKO, AS) of the last engine scan. */
 int ita_last_stats(ita_handle* h, float* kernel_ms, uint64_t* method_steps3);
 
+/* The kernel instance of the handle's last scan (no counterpart in the
+ * reference; introspection): out2[0] = the engine kernel's mode argument,
+ * the scan mode plus 0x100 for a lean kernel (rule set without word-boundary
+ * transitions, with two-byte rows, not a stream: the boundary, stream and
+ * missing-row code compiled out) and 0x200 for the narrow register profile,
+ * or -1 for the plain iNFA kernel; out2[1] = 1 for hashed-list storage. */
+int ita_last_kernel(ita_handle* h, int32_t* out2);
+
 /* Times (ms) of the handle's last scan: [0] the launch sequence (as
  * ita_last_stats), [1] the engine kernel alone (hipEvents around it; a
  * graph-captured scan replays its prefix kernels as a graph and launches the

@@ -43,8 +43,8 @@ def make_case(tmp_path, kind, seed):
         text = "\n".join(lines) + "\n"
         # random prefixes, and endings that leave accepting states active
         ends = [b"", b"wx", b"zd", b"za", b"zax", b"zbx", b"zb", b"wyy", b"zc"]
-        strs = [bytes(rng.choice(list(b"abcdxyzw "), size=int(rng.integers(0, 200)))) + ends[int(rng.integers(0, 9))]
-                for _ in range(150)]
+        strs = [rng.choice(list(b"abcdxyzw "), size=int(rng.integers(0, 200))).astype(np.uint8).tobytes()
+                + ends[int(rng.integers(0, 9))] for _ in range(150)]
     elif kind == "shift":
         text = shift_nfa(seed)
         strs = [bytes(rng.choice(list(b"abcx"), size=int(rng.integers(0, 300)))) for _ in range(150)]

@@ -89,3 +89,47 @@ def test_stream_argument_errors(tmp_path):
     assert st.scan([], []) == []
     st.close()
     sc.close()
+
+
+def no_boundary_rule_set(tmp_path):
+    # an always-active start, chains and a persistent gap, no word boundaries
+    lines = ["12", "0 -> 0 : 0|255", "0 : initial", "0 -> 1 : 97", "1 -> 2 : 98", "2 -> 2 : 0|255", "2 -> 3 : 99",
+             "3 -> 3 : 0|255", "3 : accepting 1", "0 -> 4 : 120", "4 -> 5 : 121", "5 -> 8 : 120", "8 -> 8 : 0|255",
+             "8 : accepting 2", "0 -> 6 : 97", "6 -> 6 : 97", "6 -> 7 : 98", "7 : accepting 3"]
+    path = str(tmp_path / "nowb.nfa")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    rng = np.random.default_rng(9)
+    strs = [rng.choice(list(b"abcxy "), size=int(rng.integers(0, 300))).astype(np.uint8).tobytes() for _ in range(60)]
+    return path, strs
+
+
+@pytest.mark.parametrize("kind", ["nowb", "random", "regex"])
+def test_lean_and_full_kernels_agree(tmp_path, kind):
+    # a whole-string scan of a rule set without word-boundary transitions (and
+    # with two-byte rows) runs the lean kernel, a stream always the full one;
+    # each string scanned as one chunk must give the oracle's result either way
+    path, strs = no_boundary_rule_set(tmp_path) if kind == "nowb" else rule_set(tmp_path, kind)
+    o = oracle.OracleNFA(path)
+    exp = [o.scan(s) for s in strs]
+    sc = E.Scratch(E.AD_MATCHER, path).alloc_scratch()
+    info = sc.info()
+    for mode in (E.MODE_AS, E.MODE_TKO, E.MODE_ADMATCH):
+        print(f"[case] {kind} mode={E.MODE_NAMES[mode]}", flush=True)
+        buf, off = E.pack_strings(strs)
+        d_buf, d_off = E.DeviceArray(buf), E.DeviceArray(off)
+        sc.scan_device(d_buf.ptr, d_off.ptr, len(strs), E.KIND_ENGINE, mode)
+        whole = E.unpack(*sc.collect(len(strs)))
+        kmode, store = sc.last_kernel()
+        assert (kmode & 0xff, store) == (mode, 0)
+        lean = info["wb_transitions"] == 0 and sc.table(14).size > 0
+        assert lean == (kind == "nowb")
+        assert bool(kmode & E.LEAN_BIT) == lean, (kind, kmode)
+        assert whole == exp
+    st = E.Streams(sc, len(strs))
+    got = st.scan(strs, list(range(len(strs))))
+    kmode, _ = sc.last_kernel()
+    assert not (kmode & E.LEAN_BIT)
+    assert got == exp
+    st.close()
+    sc.close()
